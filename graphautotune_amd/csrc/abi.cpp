@@ -1,0 +1,230 @@
+// Host-side C ABI: errors, schedules, graphs, plans. The GPU entry points
+// (gat_count*, gat_copy_probe, gat_device_count) live in engine.hip.
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "handles.hpp"
+
+namespace gat {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+const char *last_error() { return g_err; }
+
+}  // namespace gat
+
+using namespace gat;
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                   \
+    }                                                               \
+    catch (const std::bad_alloc &) {                                \
+        GAT_FAIL(GAT_ENOMEM, "host allocation failed");             \
+    }                                                               \
+    catch (const std::exception &ex) {                              \
+        GAT_FAIL(GAT_EINVAL, "%s", ex.what());                      \
+    }
+
+extern "C" {
+
+const char *gat_last_error(void) { return last_error(); }
+const char *gat_version(void) { return "gat-mi355x 0.1 (gfx950)"; }
+
+int gat_schedule_build(const char *adj_str, gat_schedule_data *out, size_t cap) {
+    if (!adj_str || !out) GAT_FAIL(GAT_EINVAL, "null argument");
+    if (cap < sizeof(gat_schedule_data)) GAT_FAIL(GAT_EINVAL, "schedule buffer too small (%zu < %zu)", cap,
+                                                  sizeof(gat_schedule_data));
+    GUARD_BEGIN
+    Pattern p;
+    int rc = parse_pattern(adj_str, &p);
+    if (rc) return rc;
+    Schedule s;
+    rc = build_schedule(p, &s);
+    if (rc) return rc;
+    export_schedule(s, out);
+    return GAT_OK;
+    GUARD_END
+}
+
+int gat_pattern_automorphisms(const char *adj_str, int64_t *out) {
+    if (!adj_str || !out) GAT_FAIL(GAT_EINVAL, "null argument");
+    GUARD_BEGIN
+    Pattern p;
+    int rc = parse_pattern(adj_str, &p);
+    if (rc) return rc;
+    *out = (int64_t)automorphisms(p).size();
+    return GAT_OK;
+    GUARD_END
+}
+
+static int wrap_graph(Graph &&g, gat_graph **out) {
+    auto *h = new gat_graph();
+    h->g = std::make_shared<Graph>(std::move(g));
+    *out = h;
+    return GAT_OK;
+}
+
+#define GRAPH_CALL(expr)                  \
+    if (!out) GAT_FAIL(GAT_EINVAL, "null out"); \
+    GUARD_BEGIN                           \
+    Graph g;                              \
+    int rc = (expr);                      \
+    if (rc) return rc;                    \
+    return wrap_graph(std::move(g), out); \
+    GUARD_END
+
+int gat_graph_load_bin(const char *path, gat_graph **out) { GRAPH_CALL(load_bin(path ? path : "", &g)) }
+int gat_graph_load_txt(const char *path, gat_graph **out) { GRAPH_CALL(load_txt(path ? path : "", &g)) }
+int gat_graph_gen_er(int32_t n, double p, uint64_t seed, gat_graph **out) { GRAPH_CALL(gen_er(n, p, seed, &g)) }
+int gat_graph_gen_rmat(int32_t scale, int32_t ef, double a, double b, double c, uint64_t seed, gat_graph **out) {
+    GRAPH_CALL(gen_rmat(scale, ef, a, b, c, seed, &g))
+}
+int gat_graph_gen_chung_lu(int32_t n, double mean, double gamma, int32_t maxd, uint64_t seed, gat_graph **out) {
+    GRAPH_CALL(gen_chung_lu(n, mean, gamma, maxd, seed, &g))
+}
+
+int gat_graph_from_csr(int32_t v_cnt, int64_t e, const int64_t *offsets, const int32_t *adj, gat_graph **out) {
+    if (!out || v_cnt < 0 || e < 0 || !offsets || (e > 0 && !adj)) GAT_FAIL(GAT_EINVAL, "bad CSR arguments");
+    GUARD_BEGIN
+    Graph g;
+    g.n = v_cnt;
+    g.m = e;
+    g.off.assign(offsets, offsets + v_cnt + 1);
+    g.adj.assign(adj, adj + e);
+    int rc = check_csr(g);
+    if (rc) return rc;
+    // every entry needs its reverse
+    for (int32_t v = 0; v < g.n; v++)
+        for (int64_t k = g.off[v]; k < g.off[v + 1]; k++) {
+            const int32_t w = g.adj[k];
+            auto b = g.adj.begin() + g.off[w], en = g.adj.begin() + g.off[w + 1];
+            if (!std::binary_search(b, en, v)) GAT_FAIL(GAT_EINVAL, "CSR is not symmetric (%d,%d)", v, w);
+        }
+    g.finish();
+    return wrap_graph(std::move(g), out);
+    GUARD_END
+}
+
+int gat_graph_from_edges(int32_t v_cnt, int64_t n_edges, const int32_t *src, const int32_t *dst, gat_graph **out) {
+    if (!out || n_edges < 0 || (n_edges > 0 && (!src || !dst))) GAT_FAIL(GAT_EINVAL, "bad edge-list arguments");
+    GUARD_BEGIN
+    std::vector<int32_t> s(src, src + n_edges), d(dst, dst + n_edges);
+    Graph g;
+    int rc = build_csr(v_cnt, s, d, &g);
+    if (rc) return rc;
+    return wrap_graph(std::move(g), out);
+    GUARD_END
+}
+
+int gat_graph_save_bin(const gat_graph *g, const char *path) {
+    if (!g || !path) GAT_FAIL(GAT_EINVAL, "null argument");
+    GUARD_BEGIN
+    return save_bin(*g->g, path);
+    GUARD_END
+}
+
+int gat_graph_info(const gat_graph *g, int32_t *v_cnt, int64_t *e, int32_t *max_degree) {
+    if (!g) GAT_FAIL(GAT_EINVAL, "null graph");
+    if (v_cnt) *v_cnt = g->g->n;
+    if (e) *e = g->g->m;
+    if (max_degree) *max_degree = g->g->max_deg;
+    return GAT_OK;
+}
+
+int gat_graph_csr(const gat_graph *g, const int64_t **offsets, const int32_t **adj) {
+    if (!g) GAT_FAIL(GAT_EINVAL, "null graph");
+    if (offsets) *offsets = g->g->off.data();
+    if (adj) *adj = g->g->adj.data();
+    return GAT_OK;
+}
+
+int gat_graph_degree_histogram(const gat_graph *g, int64_t *bins) {
+    if (!g || !bins) GAT_FAIL(GAT_EINVAL, "null argument");
+    std::memset(bins, 0, sizeof(int64_t) * 32);
+    for (int32_t v = 0; v < g->g->n; v++) {
+        const uint32_t d = (uint32_t)g->g->deg(v) + 1;
+        bins[31 - __builtin_clz(d)]++;
+    }
+    return GAT_OK;
+}
+
+void gat_graph_free(gat_graph *g) { delete g; }
+
+int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_patterns, int32_t mode, gat_plan **out) {
+    if (!g || !adj_strs || n_patterns < 1 || !out) GAT_FAIL(GAT_EINVAL, "bad plan arguments");
+    GUARD_BEGIN
+    std::vector<std::string> adj;
+    for (int i = 0; i < n_patterns; i++) {
+        if (!adj_strs[i]) GAT_FAIL(GAT_EINVAL, "null pattern %d", i);
+        adj.emplace_back(adj_strs[i]);
+    }
+    auto *p = new gat_plan();
+    int rc = compile_plan(adj, mode, &p->cp);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    if (p->cp.relabel) {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (!g->relabeled) {
+            auto r = std::make_shared<Graph>();
+            relabel_by_degree(*g->g, r.get(), nullptr);
+            g->relabeled = r;
+        }
+        p->graph = g->relabeled;
+    } else {
+        p->graph = g->g;
+    }
+    *out = p;
+    return GAT_OK;
+    GUARD_END
+}
+
+int gat_plan_schedule(const gat_plan *p, int32_t i, gat_schedule_data *out, size_t cap) {
+    if (!p || !out || i < 0 || i >= (int)p->cp.patterns.size()) GAT_FAIL(GAT_EINVAL, "bad arguments");
+    if (cap < sizeof(gat_schedule_data)) GAT_FAIL(GAT_EINVAL, "schedule buffer too small");
+    export_schedule(p->cp.patterns[i].sched, out);
+    return GAT_OK;
+}
+
+void gat_plan_free(gat_plan *p) { delete p; }
+
+int gat_default_config(gat_kernel_config *out) {
+    if (!out) GAT_FAIL(GAT_EINVAL, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->block_waves = 1;
+    out->waves_per_cu = 8;
+    out->queue_chunk = 16;
+    out->slot_capacity = 1024;
+    out->leaf_mode = 0;
+    out->intersect_strategy = 0;
+    out->gallop_ratio = 32;
+    out->search_linear_threshold = 0;
+    out->max_regs = 0;
+    return GAT_OK;
+}
+
+/* Debug helper: text dump of a plan's device program. */
+int gat_plan_describe(const gat_plan *p, char *buf, size_t cap) {
+    if (!p || !buf || cap == 0) GAT_FAIL(GAT_EINVAL, "bad arguments");
+    std::string s = describe_plan(p->cp.dev);
+    for (size_t i = 0; i < p->cp.patterns.size(); i++) {
+        const auto &pi = p->cp.patterns[i];
+        s += "pattern " + std::to_string(i) + " divisor " + std::to_string(pi.divisor) + " multiplier " +
+             std::to_string(pi.multiplier) + " restrictions";
+        for (auto r : pi.used_restrictions) s += " (" + std::to_string(r.first) + "," + std::to_string(r.second) + ")";
+        s += "\n";
+    }
+    std::snprintf(buf, cap, "%s", s.c_str());
+    return (int)s.size();
+}
+
+}  // extern "C"

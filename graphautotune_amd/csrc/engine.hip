@@ -1,0 +1,984 @@
+// MI355X (gfx950) subgraph pattern-matching engine.
+//
+// One persistent launch runs every pattern of a plan. Each wavefront (64
+// lanes) takes a chunk of directed edges from a global work queue; an edge
+// (v0 -> v1) fixes the first two pattern vertices, and the wave walks the
+// plan's trie depth-first from there:
+//   * non-leaf levels take one candidate at a time; their prefix sets are
+//     intersections computed by all 64 lanes (stream one sorted list,
+//     binary-search the other, compact with ballot + mbcnt) into LDS;
+//   * leaf levels count every candidate at once: the (candidate, element)
+//     pairs of all candidates form one flattened index space that the lanes
+//     sweep 64 at a time, finding their owner candidate with ds_bpermute and
+//     crediting hits with ballot + popcount, so lane utilisation does not
+//     depend on set sizes. The IEP tail is then combined per candidate.
+// Counts are exact uint64 (modular arithmetic for the signed IEP terms).
+// A task whose intersection could outgrow its LDS buffer is abandoned and
+// queued for a second launch that keeps the same buffers in HBM.
+//
+// Replaces the six-kernel host-recursive executor of the reference
+// (include/engine/engine.cuh:232-428, include/engine/engine_kernel.cuh).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "handles.hpp"
+#include "plan_dev.h"
+
+namespace gatk {
+
+constexpr int W = 64;
+
+struct KArgs {
+    const long long *off;
+    const int *adj;
+    const int *split;  // split[v] = number of neighbours of v below v
+    int n;
+    long long m;
+    const GatPlanDev *plan;
+    unsigned long long *counts;  // [n_patterns]
+    unsigned long long *queue;   // [0] main queue head, [1] overflow head
+    long long *ovf;              // overflow task starts
+    unsigned long long *ovf_n;   // overflow task count
+    long long ovf_cap;
+    int *gstore;                 // HBM set buffers (overflow launch)
+    unsigned long long *bytes;   // W_alg accumulator
+    unsigned int *err;           // device error flags
+    int cap;                     // entries per set buffer
+    int chunk;
+    int lin_thr;
+    int gallop_ratio;
+    int strategy;
+    int account;
+};
+
+struct WaveState {
+    unsigned long long task_acc[GATP_MAX_TERMS];
+    unsigned long long wave_acc[GATP_MAX_TERMS];
+    long long sbeg[GATP_MAX_SLOTS];
+    int slen[GATP_MAX_SLOTS];
+    int val[GATP_MAX_DEPTH + 1];
+    int f_node[GATP_MAX_DEPTH + 1];
+    int f_cur[GATP_MAX_DEPTH + 1];
+    int f_hi[GATP_MAX_DEPTH + 1];
+    int leafcnt[GATP_MAX_LEAF_OPS][W];
+    unsigned long long bytes;
+    int abort;
+    int pad;
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ int mbcnt(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long long uni64(long long x) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(x & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ int bperm(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
+__device__ __forceinline__ long long bperm64(long long x, int src) {
+    const int lo = bperm((int)(x & 0xffffffffll), src), hi = bperm((int)(x >> 32), src);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// LDS accesses of one wavefront are performed in program order; the fences
+// keep the compiler from moving them across this point.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int excl_scan(int x, int *total) {
+    int v = x;
+#pragma unroll
+    for (int d = 1; d < W; d <<= 1) {
+        const int y = __shfl_up(v, d);
+        if (lane_id() >= d) v += y;
+    }
+    *total = __shfl(v, W - 1);
+    return v - x;
+}
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d);
+    return x;
+}
+__device__ __forceinline__ unsigned long long range_mask(int lo, int hi) {  // bits [lo,hi), 0<=lo<hi<=64
+    const unsigned long long h = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
+    return h & ~((1ull << lo) - 1);
+}
+
+// Number of elements of sorted a[0,len) below t.
+template <class P>
+__device__ __forceinline__ int lbound(P a, int len, int t) {
+    int lo = 0, n = len;
+    while (n > 0) {
+        const int h = n >> 1;
+        if (a[lo + h] < t) {
+            lo += h + 1;
+            n -= h + 1;
+        } else {
+            n = h;
+        }
+    }
+    return lo;
+}
+// Membership of x in sorted a[0,len): linear scan below lin_thr entries
+// (the reference's set_search_type switch, array_vertex_set.cuh:145-158),
+// otherwise a branch-free binary descent that never reads past len.
+template <class P>
+__device__ __forceinline__ bool has(P a, int len, int x, int lin_thr) {
+    if (len <= 0) return false;
+    if (len < lin_thr) {
+        for (int i = 0; i < len; i++) {
+            const int y = a[i];
+            if (y >= x) return y == x;
+        }
+        return false;
+    }
+    int base = 0, n = len;
+    while (n > 1) {
+        const int half = n >> 1;
+        base = (a[base + half] <= x) ? base + half : base;
+        n -= half;
+    }
+    return a[base] == x;
+}
+
+struct Ctx {
+    const KArgs &a;
+    const GatPlanDev &P;
+    WaveState *S;
+    int *store;
+    int ln;
+};
+
+// Truncation bound of an op/vid: max over groups of min over member levels.
+// Level `d` reads the per-lane candidate c.
+__device__ __forceinline__ int group_bound(const WaveState *S, const uint8_t *groups, int ngroups, int d, int c) {
+    if (ngroups == 0) return INT_MAX;
+    int best = INT_MIN;
+    for (int g = 0; g < ngroups; g++) {
+        int mn = INT_MAX;
+        const unsigned m = groups[g];
+        for (int f = 0; f <= GATP_MAX_DEPTH; f++)
+            if (m >> f & 1u) mn = min(mn, f == d ? c : S->val[f]);
+        best = max(best, mn);
+    }
+    return best;
+}
+__device__ __forceinline__ int mask_bound(const WaveState *S, unsigned mask, int d, int c) {
+    int mn = INT_MAX;
+    for (int f = 0; f <= GATP_MAX_DEPTH; f++)
+        if (mask >> f & 1u) mn = min(mn, f == d ? c : S->val[f]);
+    return mn;
+}
+__device__ __forceinline__ bool equals_any(const WaveState *S, unsigned mask, int d, int x) {
+    bool r = false;
+    for (int f = 0; f < d; f++)
+        if (mask >> f & 1u) r |= (S->val[f] == x);
+    return r;
+}
+
+// Length of N(v) below t (split[] answers the common t == v case).
+__device__ __forceinline__ int trunc_len(const KArgs &a, int v, long long beg, int deg, int t) {
+    if (t == INT_MAX) return deg;
+    if (t == v) return a.split[v];
+    return lbound(a.adj + beg, deg, t);
+}
+
+// Wave-cooperative intersection of stream[0,slen) with search[0,rlen),
+// skipping the (at most 8) excluded values; writes the sorted result to out
+// when MAT. has_r=false means "no second operand": every element passes.
+template <bool MAT, class SP, class RP, class OP>
+__device__ __forceinline__ int wave_isect(SP stream, int slen, RP search, int rlen, bool has_r, OP out,
+                                          const int *ex, int nex, int lin_thr, int ln) {
+    int total = 0;
+    for (int base = 0; base < slen; base += W) {
+        const int i = base + ln;
+        bool hit = false;
+        int x = 0;
+        if (i < slen) {
+            x = stream[i];
+            hit = !has_r || has(search, rlen, x, lin_thr);
+            for (int k = 0; k < nex; k++) hit &= (x != ex[k]);
+        }
+        const unsigned long long m = ballot(hit);
+        if (MAT && hit) out[total + mbcnt(m)] = x;
+        total += __popcll(m);
+    }
+    return total;
+}
+
+// Stream A (the new vertex's neighbour list) or the father set?
+__device__ __forceinline__ bool stream_a(const KArgs &a, int alen, int blen, bool b_in_lds) {
+    if (a.strategy == 1) return true;
+    if (a.strategy == 2) return alen <= blen;
+    if (b_in_lds) return !((long long)alen > (long long)a.gallop_ratio * (long long)blen);
+    return alen <= blen;
+}
+
+// Calls f with the slot's content pointer; the two branches keep the LDS
+// and global provenance apart so each access compiles to ds_* or global_*.
+template <bool GS, class F>
+__device__ __forceinline__ auto with_slot(const Ctx &C, int slot, F f) {
+    const GatSlot &s = C.P.slots[slot];
+    if (s.kind == GATP_ALIAS) return f(C.a.adj + C.S->sbeg[slot]);
+    return f(C.store + (long long)s.store * C.a.cap);
+}
+
+// Evaluate one op for the wave-uniform candidate c at depth d.
+template <bool GS>
+__device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool dup) {
+    const KArgs &a = C.a;
+    WaveState *S = C.S;
+    const int t = uni(group_bound(S, o.groups, o.ngroups, d, c));
+    const long long ab = a.off[c];
+    const int deg = (int)(a.off[c + 1] - ab);
+    int alen = dup ? 0 : uni(trunc_len(a, c, ab, deg, t));
+    const int slot = o.slot;
+    if (o.kind == GATP_ALIAS) {
+        if (C.ln == 0) {
+            S->sbeg[slot] = ab;
+            S->slen[slot] = alen;
+        }
+        return;
+    }
+    int ex[8];
+    int nex = 0;
+    if (o.kind == GATP_COUNT)
+        for (int f = 0; f < d; f++)
+            if (o.excl >> f & 1u) ex[nex++] = S->val[f];
+    const int f = o.father;
+    int blen = 0, result = 0;
+    const bool has_r = f >= 0;
+    bool b_lds = false;
+    if (has_r) {
+        const GatSlot &fs = C.P.slots[f];
+        b_lds = fs.kind == GATP_LIST && !GS;
+        const int flen = S->slen[f];
+        if (fs.kind == GATP_ALIAS) {
+            const int *B = a.adj + S->sbeg[f];
+            blen = dup ? 0 : (t == INT_MAX ? flen : uni(lbound(B, flen, t)));
+        } else {
+            const int *B = C.store + (long long)fs.store * a.cap;
+            blen = dup ? 0 : (t == INT_MAX ? flen : uni(lbound(B, flen, t)));
+        }
+    }
+    if (o.kind == GATP_LIST) {
+        const int ub = has_r ? min(alen, blen) : alen;
+        if (ub > a.cap) {
+            if (C.ln == 0) S->abort = 1;
+            return;
+        }
+    }
+    const int *A = a.adj + ab;
+    int *out = C.store + (long long)(o.store < 0 ? 0 : o.store) * a.cap;
+    const bool sa = !has_r || stream_a(a, alen, blen, b_lds);
+    if (!has_r) {
+        result = o.kind == GATP_LIST ? wave_isect<true>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln)
+                                     : wave_isect<false>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln);
+    } else {
+        const GatSlot &fs = C.P.slots[f];
+        if (fs.kind == GATP_ALIAS) {
+            const int *B = a.adj + S->sbeg[f];
+            if (o.kind == GATP_LIST)
+                result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
+                            : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
+            else
+                result = sa ? wave_isect<false>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
+                            : wave_isect<false>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
+        } else {
+            const int *B = C.store + (long long)fs.store * a.cap;
+            if (o.kind == GATP_LIST)
+                result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
+                            : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
+            else
+                result = sa ? wave_isect<false>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
+                            : wave_isect<false>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
+        }
+    }
+    if (C.ln == 0) {
+        S->slen[slot] = result;
+        if (a.account) {
+            const GatSlot *fs = has_r ? &C.P.slots[f] : nullptr;
+            S->bytes += 4ull * (alen + blen) + (o.kind == GATP_LIST ? 4ull * result : 0ull) + 16ull +
+                        ((fs && fs->kind == GATP_ALIAS) ? 16ull : 0ull);
+        }
+    }
+}
+
+// Value of one IEP factor. Per-lane candidate c sits at depth d; leaf
+// counters are read for this lane.
+template <bool GS>
+__device__ __forceinline__ long long vid_value(const Ctx &C, const GatVid &v, int d, int c) {
+    const WaveState *S = C.S;
+    const GatSlot &s = C.P.slots[v.slot];
+    if (s.leaf_idx >= 0) return S->leafcnt[s.leaf_idx][C.ln];
+    const int L = S->slen[v.slot];
+    if (v.mode == GATP_VID_COUNT) return L;
+    return with_slot<GS>(C, v.slot, [&](auto p) -> long long {
+        if (v.mode == GATP_VID_LIST_REF) {
+            long long r = L;
+            for (int i = 0; i < v.refn; i++) r -= has(p, L, i == d ? c : S->val[i], C.a.lin_thr) ? 1 : 0;
+            return r;
+        }
+        const int t = v.bound ? mask_bound(S, v.bound, d, c) : INT_MAX;
+        long long r = t == INT_MAX ? L : lbound(p, L, t);
+        for (int j = 0; j <= d; j++)
+            if (v.excl >> j & 1u) {
+                const int x = j == d ? c : S->val[j];
+                if (x < t && has(p, L, x, C.a.lin_thr)) r--;
+            }
+        return r;
+    });
+}
+
+template <bool GS>
+__device__ __forceinline__ unsigned long long term_value(const Ctx &C, const GatTerm &T, int d, int c) {
+    unsigned long long local = 0, val = 1;
+    int last = -1;
+    for (int g = 0; g < T.sub_count; g++) {
+        const GatSub &sb = C.P.subs[T.sub_begin + g];
+        const unsigned long long x = (unsigned long long)vid_value<GS>(C, C.P.vids[sb.vid], d, c);
+        val = (g == last + 1) ? x : val * x;
+        if (sb.flag) {
+            last = g;
+            local += val * (unsigned long long)(long long)sb.coef;
+        }
+    }
+    return local;
+}
+
+template <bool GS>
+__device__ bool visit(const Ctx &C, const GatNode &N, int d, int c) {
+    WaveState *S = C.S;
+    if (C.ln == 0) S->val[d] = c;
+    wsync();
+    const bool dup = N.ap_levels && equals_any(S, N.ap_levels, d, c);
+    for (int k = 0; k < N.op_count; k++) {
+        exec_op_uniform<GS>(C, C.P.ops[N.op_begin + k], d, c, dup);
+        wsync();
+        if (S->abort) return false;
+    }
+    for (int k = 0; k < N.term_count; k++) {
+        const GatTerm &T = C.P.terms[N.term_begin + k];
+        const unsigned long long v = term_value<GS>(C, T, d, c);
+        if (C.ln == 0) S->task_acc[T.pattern] += v;
+    }
+    wsync();
+    return true;
+}
+
+// All candidates of a leaf frame, flattened across lanes.
+template <bool GS>
+__device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
+    const KArgs &a = C.a;
+    WaveState *S = C.S;
+    const int ln = C.ln;
+    unsigned long long bytes = 0;
+    for (int cb = cur; cb < hi; cb += W) {
+        const int ci = cb + ln;
+        const bool valid = ci < hi;
+        int c = 0;
+        if (valid) c = with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[ci]; });
+        bool pass = valid && !equals_any(S, N.excl, d, c);
+        const bool dup = pass && N.ap_levels && equals_any(S, N.ap_levels, d, c);
+        const long long ab = valid ? a.off[c] : 0;
+        const int deg = valid ? (int)(a.off[c + 1] - ab) : 0;
+        for (int k = 0; k < N.op_count; k++) {
+            const GatOp &o = C.P.ops[N.op_begin + k];
+            const int t = group_bound(S, o.groups, o.ngroups, d, c);
+            int alen = (pass && !dup) ? trunc_len(a, c, ab, deg, t) : 0;
+            if (o.kind == GATP_ALIAS) {
+                S->leafcnt[o.leaf_idx][ln] = alen;
+                continue;
+            }
+            // COUNT: |N(c) ∩ father ∩ [0,t)| minus excluded chosen vertices
+            int ex[8];
+            int nex = 0;
+            for (int f = 0; f < d; f++)
+                if (o.excl >> f & 1u) ex[nex++] = S->val[f];
+            const int f = o.father;
+            const bool has_r = f >= 0;
+            bool b_lds = false;
+            int blen = 0;
+            // Bg: father in HBM (a neighbour list, or any set in the overflow
+            // launch); Bl: father in this wave's LDS buffers.
+            const int *Bg = a.adj;
+            const int *Bl = C.store;
+            if (has_r) {
+                const GatSlot &fs = C.P.slots[f];
+                b_lds = fs.kind == GATP_LIST && !GS;
+                if (fs.kind == GATP_ALIAS) Bg = a.adj + S->sbeg[f];
+                else if (GS) Bg = C.store + (long long)fs.store * a.cap;
+                else Bl = C.store + (long long)fs.store * a.cap;
+                const int flen = S->slen[f];
+                if (pass && !dup) blen = t == INT_MAX ? flen : (b_lds ? lbound(Bl, flen, t) : lbound(Bg, flen, t));
+            }
+            const bool sa = !has_r || stream_a(a, alen, blen, b_lds);
+            const int mylen = sa ? alen : blen;
+            int T = 0;
+            const int myoff = excl_scan(mylen, &T);
+            int cnt = 0;
+            for (int k0 = 0; k0 < T; k0 += W) {
+                const int j = k0 + ln;
+                // owner: the last lane whose segment starts at or before j
+                int s = 0;
+#pragma unroll
+                for (int step = 32; step >= 1; step >>= 1)
+                    if (bperm(myoff, s + step) <= j && s + step < W) s += step;
+                const int s_off = bperm(myoff, s);
+                const int s_sa = bperm((int)sa, s);
+                const long long s_ab = bperm64(ab, s);
+                const int s_alen = bperm(alen, s);
+                const int s_blen = bperm(blen, s);
+                bool hit = false;
+                if (j < T) {
+                    const int idx = j - s_off;
+                    int x;
+                    if (s_sa) {
+                        x = a.adj[s_ab + idx];
+                        hit = !has_r || (b_lds ? has(Bl, s_blen, x, a.lin_thr) : has(Bg, s_blen, x, a.lin_thr));
+                    } else {
+                        x = b_lds ? Bl[idx] : Bg[idx];
+                        hit = has(a.adj + s_ab, s_alen, x, a.lin_thr);
+                    }
+                    for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
+                }
+                const unsigned long long m = ballot(hit);
+                const int lo = max(myoff - k0, 0), hi2 = min(myoff + mylen - k0, W);
+                if (lo < hi2) cnt += __popcll(m & range_mask(lo, hi2));
+            }
+            S->leafcnt[o.leaf_idx][ln] = cnt;
+            if (a.account && pass && !dup)
+                bytes += 4ull * (alen + blen) + 16ull +
+                         ((has_r && C.P.slots[f].kind == GATP_ALIAS) ? 16ull : 0ull);
+        }
+        wsync();
+        for (int k = 0; k < N.term_count; k++) {
+            const GatTerm &Tm = C.P.terms[N.term_begin + k];
+            unsigned long long v = pass ? term_value<GS>(C, Tm, d, c) : 0ull;
+            v = wave_sum64(v);
+            if (ln == 0) S->task_acc[Tm.pattern] += v;
+        }
+        wsync();
+    }
+    if (a.account) {
+        bytes = wave_sum64(bytes);
+        if (ln == 0) S->bytes += bytes;
+    }
+}
+
+template <bool GS>
+__device__ __forceinline__ void push_frame(const Ctx &C, int d, int node, int lo, int hi) {
+    WaveState *S = C.S;
+    const GatNode &N = C.P.nodes[node];
+    const int ls = N.loop_slot;
+    const int L = S->slen[ls];
+    int hb = L;
+    if (N.bound) {
+        const int t = mask_bound(S, N.bound, d, 0);  // bound levels are all < d
+        hb = uni(with_slot<GS>(C, ls, [&](auto p) { return lbound(p, L, t); }));
+    }
+    if (C.ln == 0) {
+        S->f_node[d] = node;
+        S->f_cur[d] = max(lo, 0);
+        S->f_hi[d] = min(hi, hb);
+    }
+    wsync();
+}
+
+template <bool GS>
+__device__ void run_task(const Ctx &C, long long e0, long long e1) {
+    const KArgs &a = C.a;
+    const GatPlanDev &P = C.P;
+    WaveState *S = C.S;
+    const GatNode &root = P.nodes[0];
+    long long e = e0;
+    while (e < e1) {
+        // row of edge e: last v with off[v] <= e
+        int lo = 0, hi = a.n;  // invariant off[lo] <= e < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.off[mid] <= e) lo = mid; else hi = mid;
+        }
+        const int v0 = uni(lo);
+        const long long rb = a.off[v0], re = a.off[v0 + 1];
+        const long long se = e1 < re ? e1 : re;
+        if (C.ln == 0) S->val[0] = v0;
+        wsync();
+        for (int k = 0; k < root.op_count; k++) exec_op_uniform<GS>(C, P.ops[root.op_begin + k], 0, v0, false);
+        wsync();
+        const int lo1 = (int)(e - rb), hi1 = (int)(se - rb);
+        int d = 1;
+        push_frame<GS>(C, 1, root.child_begin, lo1, hi1);
+        while (d >= 1) {
+            const int node = uni(S->f_node[d]);
+            const GatNode &N = P.nodes[node];
+            bool exhausted = false;
+            if (N.leaf) {
+                leaf<GS>(C, N, d, uni(S->f_cur[d]), uni(S->f_hi[d]));
+                exhausted = true;
+            } else {
+                const int cur = uni(S->f_cur[d]), fh = uni(S->f_hi[d]);
+                if (cur < fh) {
+                    const int c = uni(with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[cur]; }));
+                    if (C.ln == 0) S->f_cur[d] = cur + 1;
+                    wsync();
+                    if (N.excl && equals_any(S, N.excl, d, c)) continue;
+                    if (!visit<GS>(C, N, d, c)) return;  // abandoned (buffer too small)
+                    if (N.child_count > 0) {
+                        d++;
+                        push_frame<GS>(C, d, N.child_begin, 0, INT_MAX);
+                    }
+                    continue;
+                }
+                exhausted = true;
+            }
+            if (exhausted) {
+                if (N.next_sibling >= 0)
+                    push_frame<GS>(C, d, N.next_sibling, d == 1 ? lo1 : 0, d == 1 ? hi1 : INT_MAX);
+                else
+                    d--;
+            }
+        }
+        e = se;
+    }
+}
+
+template <bool GS>
+__global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wid = threadIdx.x / W, ln = threadIdx.x % W;
+    const int wpb = blockDim.x / W;
+    WaveState *S = reinterpret_cast<WaveState *>(smem + (size_t)wid * wave_bytes);
+    const GatPlanDev &P = *a.plan;
+    int *store;
+    if (GS)
+        store = a.gstore + (long long)(blockIdx.x * wpb + wid) * P.n_store * (long long)a.cap;
+    else
+        store = reinterpret_cast<int *>(smem + (size_t)wid * wave_bytes + sizeof(WaveState));
+    Ctx C{a, P, S, store, ln};
+    const int np = P.n_patterns;
+    for (int i = ln; i < np; i += W) S->wave_acc[i] = 0;
+    if (ln == 0) S->bytes = 0;
+    wsync();
+    for (;;) {
+        long long e0, e1;
+        if (GS) {
+            unsigned long long r = 0;
+            if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
+            r = (unsigned long long)uni64((long long)__shfl(r, 0));
+            const unsigned long long total = *a.ovf_n;
+            if (r >= total || (long long)r >= a.ovf_cap) break;
+            e0 = a.ovf[r];
+        } else {
+            unsigned long long r = 0;
+            if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
+            e0 = uni64((long long)__shfl(r, 0));
+            if (e0 >= a.m) break;
+        }
+        e1 = e0 + a.chunk < a.m ? e0 + a.chunk : a.m;
+        for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
+        if (ln == 0) S->abort = 0;
+        wsync();
+        run_task<GS>(C, e0, e1);
+        wsync();
+        if (S->abort) {
+            if (GS) {
+                if (ln == 0) atomicOr(a.err, 1u);  // cannot happen: HBM buffers hold max degree
+            } else if (ln == 0) {
+                const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
+                if ((long long)idx < a.ovf_cap) a.ovf[idx] = e0; else atomicOr(a.err, 2u);
+            }
+        } else {
+            for (int i = ln; i < np; i += W) S->wave_acc[i] += S->task_acc[i];
+        }
+        wsync();
+    }
+    for (int i = ln; i < np; i += W)
+        if (S->wave_acc[i]) atomicAdd(&a.counts[i], S->wave_acc[i]);
+    if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
+}
+
+__global__ void fill_u64(unsigned long long *p, long long n, unsigned long long v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void split_kernel(const long long *off, const int *adj, int n, int *split) {
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x)
+        split[v] = lbound(adj + off[v], (int)(off[v + 1] - off[v]), (int)v);
+}
+
+__global__ void copy_kernel(const int4 *__restrict__ src, int4 *__restrict__ dst, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace gatk
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace gat {
+
+#define HIPCHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) GAT_FAIL(GAT_ENODEV, "%s failed: %s", #x, hipGetErrorString(e_));      \
+    } while (0)
+
+struct DeviceInfo {
+    bool ok = false;
+    int cus = 0;
+    size_t lds_per_block = 0;
+    char arch[64] = {};
+};
+
+static int device_info(DeviceInfo *out) {
+    static DeviceInfo info;
+    static std::once_flag once;
+    static int rc = GAT_OK;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            set_error("no HIP device visible");
+            rc = GAT_ENODEV;
+            return;
+        }
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, 0) != hipSuccess) {
+            set_error("hipGetDeviceProperties failed");
+            rc = GAT_ENODEV;
+            return;
+        }
+        info.cus = p.multiProcessorCount;
+        info.lds_per_block = p.sharedMemPerBlock;
+        std::snprintf(info.arch, sizeof(info.arch), "%s", p.gcnArchName);
+        if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+            set_error("libgat is built for gfx950; device is %s", p.gcnArchName);
+            rc = GAT_ENODEV;
+            return;
+        }
+        info.ok = true;
+    });
+    if (rc) {
+        set_error("no usable gfx950 device");
+        return rc;
+    }
+    *out = info;
+    return GAT_OK;
+}
+
+struct DeviceGraph {
+    long long *off = nullptr;
+    int *adj = nullptr;
+    int *split = nullptr;
+    ~DeviceGraph() {
+        if (off) (void)hipFree(off);
+        if (adj) (void)hipFree(adj);
+        if (split) (void)hipFree(split);
+    }
+};
+
+struct PlanDevice {
+    hipStream_t stream = nullptr;
+    GatPlanDev *plan = nullptr;
+    unsigned long long *counts = nullptr, *queue = nullptr, *ovf_n = nullptr, *bytes = nullptr;
+    unsigned int *err = nullptr;
+    long long *ovf = nullptr;
+    long long ovf_cap = 0;
+    int *gstore = nullptr;
+    size_t gstore_bytes = 0;
+    int ovf_grid = 0;
+    hipEvent_t ev[4] = {};
+    std::shared_ptr<DeviceGraph> dg;
+    // captured graph
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+void destroy_plan_device(PlanDevice *d) {
+    if (!d) return;
+    if (d->gexec) (void)hipGraphExecDestroy(d->gexec);
+    if (d->graph) (void)hipGraphDestroy(d->graph);
+    for (auto &e : d->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
+                    (void *)d->err, (void *)d->ovf, (void *)d->gstore})
+        if (p) (void)hipFree(p);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+static int ensure_device_graph(Graph &g, std::shared_ptr<DeviceGraph> *out) {
+    if (g.dev) {
+        *out = std::static_pointer_cast<DeviceGraph>(g.dev);
+        return GAT_OK;
+    }
+    auto dg = std::make_shared<DeviceGraph>();
+    HIPCHECK(hipMalloc(&dg->off, sizeof(long long) * (size_t(g.n) + 1)));
+    HIPCHECK(hipMalloc(&dg->adj, sizeof(int) * std::max<size_t>(1, size_t(g.m))));
+    HIPCHECK(hipMalloc(&dg->split, sizeof(int) * std::max<size_t>(1, size_t(g.n))));
+    HIPCHECK(hipMemcpy(dg->off, g.off.data(), sizeof(long long) * (size_t(g.n) + 1), hipMemcpyHostToDevice));
+    if (g.m) HIPCHECK(hipMemcpy(dg->adj, g.adj.data(), sizeof(int) * size_t(g.m), hipMemcpyHostToDevice));
+    if (g.n) {
+        hipLaunchKernelGGL(gatk::split_kernel, dim3(1024), dim3(256), 0, nullptr, dg->off, dg->adj, g.n, dg->split);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipDeviceSynchronize());
+    g.dev = dg;
+    *out = dg;
+    return GAT_OK;
+}
+
+struct Launch {
+    gatk::KArgs a;
+    dim3 grid, block;
+    size_t lds = 0;
+    int wave_bytes = 0;
+    dim3 ovf_grid;
+    size_t ovf_lds = 0;
+    int ovf_wave_bytes = 0;
+    gatk::KArgs ao;
+};
+
+static void fill_defaults(gat_kernel_config *c) {
+    if (c->block_waves <= 0) c->block_waves = 1;
+    if (c->waves_per_cu <= 0) c->waves_per_cu = 8;
+    if (c->queue_chunk <= 0) c->queue_chunk = 16;
+    if (c->slot_capacity <= 0) c->slot_capacity = 1024;
+    if (c->intersect_strategy < 0 || c->intersect_strategy > 2) c->intersect_strategy = 0;
+    if (c->gallop_ratio <= 0) c->gallop_ratio = 32;
+    if (c->search_linear_threshold < 0) c->search_linear_threshold = 0;
+}
+
+static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc) return rc;
+    gat_kernel_config cfg{};
+    if (cfg_in) cfg = *cfg_in;
+    fill_defaults(&cfg);
+    if (cfg.block_waves > 4) GAT_FAIL(GAT_EINFEASIBLE, "block_waves %d > 4", cfg.block_waves);
+    Graph &g = *p->graph;
+    const GatPlanDev &pd = p->cp.dev;
+    if (!p->dev) {
+        auto *d = new PlanDevice();
+        p->dev = d;
+        rc = ensure_device_graph(g, &d->dg);
+        if (rc) return rc;
+        HIPCHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+        HIPCHECK(hipMalloc(&d->plan, sizeof(GatPlanDev)));
+        HIPCHECK(hipMemcpy(d->plan, &pd, sizeof(GatPlanDev), hipMemcpyHostToDevice));
+        HIPCHECK(hipMalloc(&d->counts, sizeof(unsigned long long) * GATP_MAX_TERMS));
+        HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 4));
+        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 2));
+        HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 2));
+        HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
+        for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
+        // HBM buffers for the overflow launch: every set fits in max degree
+        d->ovf_grid = std::max(64, di.cus);
+        d->gstore_bytes = (size_t)d->ovf_grid * std::max(1, pd.n_store) * (size_t)std::max(1, g.max_deg) * 4;
+        HIPCHECK(hipMalloc(&d->gstore, std::max<size_t>(d->gstore_bytes, 64)));
+    }
+    PlanDevice *d = p->dev;
+    const long long chunk = cfg.queue_chunk;
+    const long long ovf_cap = (g.m + chunk - 1) / chunk + 1;
+    if (ovf_cap > d->ovf_cap) {
+        if (d->ovf) HIPCHECK(hipFree(d->ovf));
+        HIPCHECK(hipMalloc(&d->ovf, sizeof(long long) * ovf_cap));
+        d->ovf_cap = ovf_cap;
+    }
+    const int wave_bytes = int((sizeof(gatk::WaveState) + size_t(pd.n_store) * cfg.slot_capacity * 4 + 15) / 16 * 16);
+    const size_t lds = size_t(wave_bytes) * cfg.block_waves;
+    if (lds > di.lds_per_block)
+        GAT_FAIL(GAT_EINFEASIBLE, "config needs %zu B of LDS per workgroup (device allows %zu)", lds,
+                 di.lds_per_block);
+    gatk::KArgs a{};
+    a.off = d->dg->off;
+    a.adj = d->dg->adj;
+    a.split = d->dg->split;
+    a.n = g.n;
+    a.m = g.m;
+    a.plan = d->plan;
+    a.counts = d->counts;
+    a.queue = d->queue;
+    a.ovf = d->ovf;
+    a.ovf_n = d->ovf_n;
+    a.ovf_cap = d->ovf_cap;
+    a.gstore = d->gstore;
+    a.bytes = d->bytes;
+    a.err = d->err;
+    a.cap = cfg.slot_capacity;
+    a.chunk = cfg.queue_chunk;
+    a.lin_thr = cfg.search_linear_threshold;
+    a.gallop_ratio = cfg.gallop_ratio;
+    a.strategy = cfg.intersect_strategy;
+    a.account = 0;
+    L->a = a;
+    L->block = dim3(64 * cfg.block_waves);
+    const int total_waves = std::max(1, cfg.waves_per_cu * di.cus);
+    L->grid = dim3(std::max(1, total_waves / cfg.block_waves));
+    L->lds = lds;
+    L->wave_bytes = wave_bytes;
+    L->ao = a;
+    L->ao.cap = std::max(1, g.max_deg);
+    L->ovf_grid = dim3(d->ovf_grid);
+    L->ovf_wave_bytes = int((sizeof(gatk::WaveState) + 15) / 16 * 16);
+    L->ovf_lds = L->ovf_wave_bytes;
+    return GAT_OK;
+}
+
+// Enqueue the whole count on the plan's stream (capturable: no memsets).
+static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
+    hipStream_t s = d->stream;
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->counts, (long long)GATP_MAX_TERMS, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 4ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 2ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->bytes, 2ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
+    gatk::KArgs a = L.a, ao = L.ao;
+    a.account = ao.account = account ? 1 : 0;
+    if (events) HIPCHECK(hipEventRecord(d->ev[0], s));
+    hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, a, L.wave_bytes);
+    if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
+    hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+    if (events) HIPCHECK(hipEventRecord(d->ev[2], s));
+    HIPCHECK(hipGetLastError());
+    return GAT_OK;
+}
+
+static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
+    PlanDevice *d = p->dev;
+    const int np = p->cp.dev.n_patterns;
+    unsigned long long raw[GATP_MAX_TERMS], ovf = 0, bytes = 0;
+    unsigned int err = 0;
+    HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipMemcpyAsync(&ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipMemcpyAsync(&bytes, d->bytes, sizeof(bytes), hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipMemcpyAsync(&err, d->err, sizeof(err), hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipStreamSynchronize(d->stream));
+    if (err) GAT_FAIL(GAT_ENODEV, "engine reported internal error flags 0x%x", err);
+    for (int i = 0; i < np; i++) {
+        const PatternInfo &pi = p->cp.patterns[i];
+        if (raw[i] % pi.divisor)
+            GAT_FAIL(GAT_EINVAL, "internal: raw count %llu of pattern %d not divisible by %llu", raw[i], i,
+                     (unsigned long long)pi.divisor);
+        if (counts_out) counts_out[i] = raw[i] / pi.divisor * pi.multiplier;
+    }
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        if (timed) {
+            float ms = 0, ms_main = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, d->ev[0], d->ev[2]));
+            HIPCHECK(hipEventElapsedTime(&ms_main, d->ev[0], d->ev[1]));
+            st->seconds = ms / 1000.0;
+            st->main_kernel_ms = ms_main;
+        }
+        st->overflow_tasks = ovf;
+        st->alg_bytes = bytes;
+        st->tasks = (uint64_t)p->graph->m;
+        st->grid_blocks = (int)L.grid.x;
+        st->block_threads = (int)L.block.x;
+        st->lds_bytes = (int)L.lds;
+    }
+    return GAT_OK;
+}
+
+}  // namespace gat
+
+using namespace gat;
+
+extern "C" int gat_device_count(int32_t *out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (out) *out = n;
+    return GAT_OK;
+}
+
+extern "C" int gat_count(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags, uint64_t *counts_out,
+                         gat_count_stats *stats) {
+    if (!p) GAT_FAIL(GAT_EINVAL, "null plan");
+    Launch L;
+    int rc = prepare(p, cfg, &L);
+    if (rc) return rc;
+    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true);
+    if (rc) return rc;
+    return finish(p, L, counts_out, stats, true);
+}
+
+extern "C" int gat_count_graph(gat_plan *p, const gat_kernel_config *cfg, int32_t replays, uint64_t *counts_out,
+                               gat_count_stats *stats) {
+    if (!p || replays < 1) GAT_FAIL(GAT_EINVAL, "null plan or replays < 1");
+    Launch L;
+    int rc = prepare(p, cfg, &L);
+    if (rc) return rc;
+    PlanDevice *d = p->dev;
+    if (d->gexec) {
+        (void)hipGraphExecDestroy(d->gexec);
+        d->gexec = nullptr;
+    }
+    if (d->graph) {
+        (void)hipGraphDestroy(d->graph);
+        d->graph = nullptr;
+    }
+    HIPCHECK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(d, L, false, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(d->stream, &graph);
+    if (rc) return rc;
+    if (ec != hipSuccess) GAT_FAIL(GAT_ENODEV, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    d->graph = graph;
+    HIPCHECK(hipGraphInstantiate(&d->gexec, graph, nullptr, nullptr, 0));
+    for (int r = 0; r < replays; r++) {
+        HIPCHECK(hipEventRecord(d->ev[0], d->stream));
+        HIPCHECK(hipGraphLaunch(d->gexec, d->stream));
+        HIPCHECK(hipEventRecord(d->ev[2], d->stream));
+        if (r + 1 < replays) HIPCHECK(hipStreamSynchronize(d->stream));
+    }
+    rc = finish(p, L, counts_out, stats, false);
+    if (rc) return rc;
+    if (stats) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, d->ev[0], d->ev[2]));
+        stats->seconds = ms / 1000.0;
+    }
+    return GAT_OK;
+}
+
+extern "C" int gat_copy_probe(uint64_t bytes, int32_t reps, double *gbps_out) {
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc) return rc;
+    if (bytes < 16 || reps < 1) GAT_FAIL(GAT_EINVAL, "copy probe needs bytes >= 16 and reps >= 1");
+    const long long n = (long long)(bytes / 16);
+    int4 *src = nullptr, *dst = nullptr;
+    HIPCHECK(hipMalloc(&src, n * 16));
+    HIPCHECK(hipMalloc(&dst, n * 16));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    const int grid = di.cus * 8;
+    hipLaunchKernelGGL(gatk::copy_kernel, dim3(grid), dim3(256), 0, nullptr, src, dst, n);
+    HIPCHECK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(gatk::copy_kernel, dim3(grid), dim3(256), 0, nullptr, src, dst, n);
+    HIPCHECK(hipEventRecord(e1, nullptr));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (gbps_out) *gbps_out = 2.0 * n * 16 * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return GAT_OK;
+}

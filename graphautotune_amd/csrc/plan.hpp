@@ -1,0 +1,40 @@
+// Compiles Schedules into the device program of plan_dev.h.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pattern.hpp"
+#include "plan_dev.h"
+
+namespace gat {
+
+struct PatternInfo {
+    std::string adj;
+    Schedule sched;
+    int mode = GAT_MODE_UNIQUE;
+    // The engine's raw total equals divisor * (requested count) before the
+    // multiplier: unique = raw / divisor, labeled = unique * |Aut|.
+    uint64_t divisor = 1;
+    uint64_t multiplier = 1;
+    std::vector<Pair> used_restrictions;  // restrictions the engine applies
+};
+
+struct CompiledPlan {
+    GatPlanDev dev;
+    std::vector<PatternInfo> patterns;
+    bool relabel = false;  // run on the degree-relabelled graph
+};
+
+// For unique counting: picks the restriction set to apply and the constant
+// factor c such that raw count = c * #instances, by checking that every
+// ordering of an instance's vertex ids admits the same number of
+// automorphic embeddings. Returns c (>= 1).
+uint64_t unique_restrictions(const Schedule &s, std::vector<Pair> *used);
+
+// mode: GAT_MODE_UNIQUE / GAT_MODE_REFERENCE / GAT_MODE_LABELED
+int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *out);
+
+std::string describe_plan(const GatPlanDev &d);
+
+}  // namespace gat

@@ -1,0 +1,223 @@
+/*
+ * gat.h -- C ABI of the MI355X subgraph pattern-matching engine (libgat.so).
+ *
+ * The reference has no FFI: its boundary is an in-process C++ template API
+ * plus a process/file protocol. Each entry point below names the reference
+ * interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - Every function returns 0 (GAT_OK) on success and a negative GAT_E*
+ *     code on failure; the message is available from gat_last_error() on the
+ *     same thread. Nothing here calls exit(); the reference's gpuErrchk and
+ *     graph-mismatch paths exit the process instead
+ *     (include/utils/cuda_utils.cuh:9-16, include/infra/graph_backend.cuh:41-45).
+ *   - Vertex ids are int32, CSR offsets int64 (include/core/types.hpp:4-5).
+ *     Counts are uint64 (the reference uses long long, target/pm.cu:62).
+ *   - A handle is used by one host thread at a time. Each gat_plan owns one
+ *     HIP stream; the GPU entry points need a visible gfx950 device and fail
+ *     with GAT_ENODEV otherwise (there is no CPU fallback).
+ */
+#ifndef GAT_H
+#define GAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define GAT_OK 0
+#define GAT_EINVAL (-1)      /* bad argument / malformed pattern or graph  */
+#define GAT_EIO (-2)         /* file could not be read or written          */
+#define GAT_ENOMEM (-3)      /* host or device allocation failed           */
+#define GAT_ENODEV (-4)      /* no usable GPU / HIP runtime error          */
+#define GAT_ELIMIT (-5)      /* a fixed capacity would be exceeded         */
+#define GAT_EUNSUPPORTED (-6)/* the request has no defined result          */
+#define GAT_EINFEASIBLE (-7) /* kernel config cannot launch (LDS, block)   */
+
+/* ---- capacities of the schedule record ------------------------------ */
+/* MAX_VERTEXES / MAX_PREFIXS of include/consts/general_consts.hpp:3-4 are
+ * 10 / 10; the prefix capacity is raised to 16 because some 6-vertex
+ * schedules need 11-12 prefixes (the reference overflows silently). */
+#define GAT_MAX_VERTEXES 10
+#define GAT_MAX_PREFIXS 16
+#define GAT_MAX_IEP_GROUPS 64 /* schedule.hpp:962 has 50 */
+
+/* Counting semantics (see DESIGN.md "Count semantics").
+ *   GAT_MODE_UNIQUE    : number of distinct non-induced subgraphs of the
+ *                        data graph isomorphic to the pattern.
+ *   GAT_MODE_REFERENCE : the value the reference engine computes for the
+ *                        same schedule (include/engine/engine_kernel.cuh,
+ *                        include/core/schedule.hpp:828-831, :955); k*#K_k
+ *                        for a k-clique.
+ *   GAT_MODE_LABELED   : labeled embeddings (|Aut| * unique), the quantity
+ *                        target/naive_pm.cu:10-44 counts. */
+#define GAT_MODE_UNIQUE 0
+#define GAT_MODE_REFERENCE 1
+#define GAT_MODE_LABELED 2
+
+/* ---- opaque handles -------------------------------------------------- */
+typedef struct gat_graph gat_graph; /* host CSR (+ device copy once used) */
+typedef struct gat_plan gat_plan;   /* compiled schedules + device state   */
+
+/* Flat, pointer-free schedule record. Field-for-field superset of
+ * Core::ScheduleData (include/core/schedule.hpp:984-1026); the extra fields
+ * carry what the engine needs beyond the reference's record (the chosen
+ * permutation, the unfiltered restriction list, the automorphism count). */
+typedef struct gat_schedule_data {
+    int32_t v_cnt;                  /* pattern vertices                   */
+    int32_t iep_suffix_vertexes;    /* schedule.hpp:595                   */
+    int32_t basic_vertexes;         /* schedule.hpp:597                   */
+    int32_t total_prefix_num;
+    int32_t total_restrictions_num; /* after the second<basic filter      */
+    int32_t prefix_depth[GAT_MAX_PREFIXS];
+    int32_t prefix_data[GAT_MAX_PREFIXS][GAT_MAX_VERTEXES];
+    int32_t prefixs_father[GAT_MAX_PREFIXS];
+    int32_t prefixs_size_only[GAT_MAX_PREFIXS];
+    int32_t vertex_prefix_start[GAT_MAX_VERTEXES + 1];
+    int32_t loop_set_prefix_id[GAT_MAX_VERTEXES];
+    int32_t restrictions_start[GAT_MAX_VERTEXES + 1];
+    int32_t restrictions[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES];        /* first  */
+    int32_t restriction_second[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES]; /* second */
+    int32_t iep_prefix_num;
+    int32_t subgroups_num;
+    int32_t iep_vertex_id[GAT_MAX_PREFIXS];
+    int32_t iep_ans_pos[GAT_MAX_IEP_GROUPS];
+    int32_t iep_coef[GAT_MAX_IEP_GROUPS];
+    int32_t iep_flag[GAT_MAX_IEP_GROUPS];
+    /* extras */
+    int32_t perm[GAT_MAX_VERTEXES];  /* input vertex i -> schedule vertex perm[i] */
+    int32_t n_all_restrictions;      /* best restriction set, unfiltered  */
+    int32_t all_restr_first[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES];
+    int32_t all_restr_second[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES];
+    int32_t automorphisms;           /* |Aut(P)|                          */
+    uint32_t adj_rows[GAT_MAX_VERTEXES]; /* permuted pattern, bit j of row i */
+    double best_cost;                /* estimate_restrictions of the pick */
+} gat_schedule_data;
+
+/* Runtime kernel configuration. The reference fixes these at compile time
+ * (include/configs/engine_config.hpp, vertex_set_config.hpp,
+ * include/generated/default_config.hpp:5); here they
+ * are runtime values, and variants that must be compile-time are
+ * pre-instantiated. Zero in any field selects the engine default. */
+typedef struct gat_kernel_config {
+    int32_t block_waves;      /* waves per workgroup (threads_per_block/64)   */
+    int32_t waves_per_cu;     /* persistent grid = waves_per_cu * CUs          */
+    int32_t queue_chunk;      /* directed edges taken per work-queue grab      */
+    int32_t slot_capacity;    /* int32 entries per LDS prefix-set slot         */
+    int32_t leaf_mode;        /* 0 flattened, else lanes per candidate 1..64
+                                 (threads_per_warp, engine_config.hpp:6)      */
+    int32_t intersect_strategy; /* 0 auto, 1 stream-and-search, 2 gallop      */
+    int32_t gallop_ratio;     /* auto: gallop when |long| > ratio * |short|   */
+    int32_t search_linear_threshold; /* set_search_type: linear scan below this
+                                 size, binary above (array_vertex_set.cuh:145) */
+    int32_t max_regs;         /* register cap hint: 0, 64 or 128 VGPRs         */
+    int32_t reserved[3];
+} gat_kernel_config;
+
+/* Per-call statistics filled by gat_count (any pointer may be NULL). */
+typedef struct gat_count_stats {
+    double seconds;           /* device time of the count (hipEvents)        */
+    double main_kernel_ms;    /* average duration of the main engine launch  */
+    uint64_t overflow_tasks;  /* tasks re-run with HBM-resident prefix sets  */
+    uint64_t alg_bytes;       /* W_alg, only when GAT_FLAG_ACCOUNT is set    */
+    uint64_t tasks;           /* directed edges in the task space            */
+    int32_t grid_blocks;      /* launched workgroups                         */
+    int32_t block_threads;
+    int32_t lds_bytes;        /* dynamic LDS per workgroup                   */
+    int32_t reserved;
+} gat_count_stats;
+
+#define GAT_FLAG_ACCOUNT 1u   /* instrumented pass: accumulate W_alg bytes  */
+
+/* ---- errors ---------------------------------------------------------- */
+const char *gat_last_error(void);
+const char *gat_version(void);
+
+/* ---- patterns and schedules ----------------------------------------- */
+/* Replaces Core::Pattern{adj_str} + Core::Schedule{pattern}
+ * (include/core/pattern.hpp:22-39, include/core/schedule.hpp:864-958) and
+ * the ScheduleData record (schedule.hpp:984-1026). `out` must point to a
+ * gat_schedule_data; `cap` is its size in bytes. */
+int gat_schedule_build(const char *adj_str, gat_schedule_data *out, size_t cap);
+
+/* |Aut(P)| of a pattern string (schedule.hpp:189-191). */
+int gat_pattern_automorphisms(const char *adj_str, int64_t *out);
+
+/* ---- graphs ---------------------------------------------------------- */
+/* Replaces Infra::GlobalMemoryGraph(file, binary=true)
+ * (include/infra/graph_backend.cuh:91-99,155-161): int32 v_cnt, int64 e_cnt
+ * (directed entries), int64 offsets[v_cnt+1], int32 adj[e_cnt]. */
+int gat_graph_load_bin(const char *path, gat_graph **out);
+/* Text form of graph_backend.cuh:23-89: "v_cnt e_cnt" then e_cnt lines
+ * "u v"; ids relabelled by sorted value, self-loops dropped, symmetrised,
+ * sorted, deduplicated. */
+int gat_graph_load_txt(const char *path, gat_graph **out);
+/* graph_backend.cuh:163-169 */
+int gat_graph_save_bin(const gat_graph *g, const char *path);
+/* Copy an existing CSR (offsets[v_cnt+1], adj[e_directed]); the adjacency
+ * rows must be sorted, free of duplicates and self-loops, and symmetric. */
+int gat_graph_from_csr(int32_t v_cnt, int64_t e_directed, const int64_t *offsets,
+                       const int32_t *adj, gat_graph **out);
+/* Build from an undirected edge list (any order, duplicates and self-loops
+ * allowed): symmetrised, sorted, deduplicated. */
+int gat_graph_from_edges(int32_t v_cnt, int64_t n_edges, const int32_t *src,
+                         const int32_t *dst, gat_graph **out);
+/* Seeded generators (splitmix64-seeded xoshiro256**). utils/random_graph.cpp
+ * draws from unseeded rand(); these replace it with reproducible streams. */
+int gat_graph_gen_er(int32_t n, double p, uint64_t seed, gat_graph **out);
+int gat_graph_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                       uint64_t seed, gat_graph **out);
+int gat_graph_gen_chung_lu(int32_t n, double mean_degree, double gamma, int32_t max_degree,
+                           uint64_t seed, gat_graph **out);
+int gat_graph_info(const gat_graph *g, int32_t *v_cnt, int64_t *e_directed,
+                   int32_t *max_degree);
+/* Read-only views of the host CSR (valid until gat_graph_free). */
+int gat_graph_csr(const gat_graph *g, const int64_t **offsets, const int32_t **adj);
+/* 32-bin log2-degree histogram (bin k counts vertices with
+ * floor(log2(deg+1)) == k), used as cost-model features. */
+int gat_graph_degree_histogram(const gat_graph *g, int64_t *bins32);
+void gat_graph_free(gat_graph *g);
+
+/* ---- plans and counting --------------------------------------------- */
+/* Replaces Engine::DeviceContext{schedule, graph} + context.to_device() +
+ * Engine::Executor{set_size, GPU_DEVICE} (include/engine/context.cuh:18-25,
+ * include/engine/engine.cuh:157-204) for n_patterns schedules at once.
+ * Host-side work only (schedule build, plan compilation, relabelling);
+ * device buffers are created on the first gat_count. */
+int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_patterns,
+                    int32_t mode, gat_plan **out);
+/* Number of patterns and the schedule record of pattern i. */
+int gat_plan_schedule(const gat_plan *p, int32_t i, gat_schedule_data *out, size_t cap);
+/* Replaces Executor::perform_search (engine.cuh:232-265) for every pattern
+ * of the plan in one persistent launch. counts_out has n_patterns entries.
+ * The timed span matches target/pm.cu:60-68 (the search only). */
+int gat_count(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
+              uint64_t *counts_out, gat_count_stats *stats);
+/* Capture the count sequence of `cfg` into a hipGraph once, then replay it
+ * `replays` times; counts and stats are those of the last replay. */
+int gat_count_graph(gat_plan *p, const gat_kernel_config *cfg, int32_t replays,
+                    uint64_t *counts_out, gat_count_stats *stats);
+void gat_plan_free(gat_plan *p);
+
+/* Text dump of a plan's compiled device program (debugging aid). Returns
+ * the full length of the text (which may exceed cap) or an error code. */
+int gat_plan_describe(const gat_plan *p, char *buf, size_t cap);
+
+/* Default configuration the engine uses for zero fields. */
+int gat_default_config(gat_kernel_config *out);
+
+/* ---- device probes --------------------------------------------------- */
+/* Number of visible HIP devices (0 when none); never initialises more than
+ * hipGetDeviceCount does. */
+int gat_device_count(int32_t *out);
+/* Plain device-to-device copy of `bytes` through a HIP kernel, timed with
+ * hipEvents; used to scale the HBM traffic counters. */
+int gat_copy_probe(uint64_t bytes, int32_t reps, double *gbps_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GAT_H */

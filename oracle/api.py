@@ -1,0 +1,103 @@
+"""ctypes wrapper of liboracle.so (checker only; see oracle.h)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            from . import build
+            build.build()
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        L.orc_schedule.argtypes = [C.c_char_p, P]
+        L.orc_schedule.restype = C.c_int
+        L.orc_automorphisms.argtypes = [C.c_char_p]
+        L.orc_automorphisms.restype = C.c_int64
+        L.orc_graph_cnt.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.orc_graph_cnt.restype = None
+        for f in (L.orc_naive_count, L.orc_labeled_count):
+            f.argtypes = [C.c_int32, P, P, C.c_char_p]
+            f.restype = C.c_uint64
+        L.orc_reference_count.argtypes = [C.c_int32, P, P, P, C.POINTER(C.c_int)]
+        L.orc_reference_count.restype = C.c_uint64
+        L.orc_clique_walg.argtypes = [C.c_int32, P, P, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_clique_walg.restype = C.c_uint64
+        L.orc_relabel.argtypes = [C.c_int32, P, P, P, P]
+        L.orc_relabel.restype = None
+        _lib = L
+    return _lib
+
+
+def _csr(off, adj):
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    adj = np.ascontiguousarray(adj, dtype=np.int32)
+    return off, adj, len(off) - 1
+
+
+def schedule(adj_str, struct_type):
+    """Restated Core::Schedule; returns a struct_type (gat_schedule_data layout)."""
+    d = struct_type()
+    rc = lib().orc_schedule(adj_str.encode(), C.byref(d))
+    if rc != 0:
+        raise ValueError("oracle schedule failed (%d) for %s" % (rc, adj_str))
+    return d
+
+
+def automorphisms(adj_str):
+    return int(lib().orc_automorphisms(adj_str.encode()))
+
+
+def graph_cnt(k):
+    e, o = C.c_int64(), C.c_int64()
+    lib().orc_graph_cnt(k, C.byref(e), C.byref(o))
+    return e.value, o.value
+
+
+def naive_count(off, adj, pattern):
+    off, adj, n = _csr(off, adj)
+    return int(lib().orc_naive_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, pattern.encode()))
+
+
+def labeled_count(off, adj, pattern):
+    off, adj, n = _csr(off, adj)
+    return int(lib().orc_labeled_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, pattern.encode()))
+
+
+def unique_count(off, adj, pattern):
+    lab = labeled_count(off, adj, pattern)
+    aut = automorphisms(pattern)
+    assert lab % aut == 0, (lab, aut)
+    return lab // aut
+
+
+def reference_count(off, adj, sched_struct):
+    """Reference engine value (long long bits as uint64) and a defined flag."""
+    off, adj, n = _csr(off, adj)
+    ok = C.c_int()
+    v = lib().orc_reference_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None,
+                                  C.byref(sched_struct), C.byref(ok))
+    return int(v), bool(ok.value)
+
+
+def relabel(off, adj):
+    off, adj, n = _csr(off, adj)
+    o2 = np.zeros(n + 1, dtype=np.int64)
+    a2 = np.zeros(max(1, len(adj)), dtype=np.int32)
+    lib().orc_relabel(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, o2.ctypes.data, a2.ctypes.data)
+    return o2, a2[:len(adj)]
+
+
+def clique_walg(off_relabeled, adj_relabeled, k):
+    off, adj, n = _csr(off_relabeled, adj_relabeled)
+    w = C.c_uint64()
+    c = lib().orc_clique_walg(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, k, C.byref(w))
+    return int(c), int(w.value)

@@ -1,0 +1,55 @@
+/*
+ * CPU oracle for the subgraph pattern-matching path. TEST INFRASTRUCTURE
+ * ONLY: it is loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker, never by the product.
+ *
+ * Every function restates a reference algorithm (paths relative to the
+ * reference repository root):
+ *   orc_schedule            include/core/schedule.hpp:17-1026 (literal
+ *                           restatement, brute force where the reference is)
+ *   orc_naive_count         target/naive_pm.cu:10-44 (uint64 instead of int)
+ *   orc_reference_count     the reference engine's count for a schedule:
+ *                           include/engine/engine.cuh:232-428 and
+ *                           include/engine/engine_kernel.cuh:40-432, with
+ *                           include/implementations/array_vertex_set.cuh
+ *   orc_labeled_count       labeled (injective, edge-preserving, non-induced)
+ *                           embeddings by backtracking; = naive_pm's value
+ *   orc_automorphisms       schedule.hpp:173-191
+ *   orc_graph_cnt           schedule.hpp:86-170
+ *   orc_clique_walg         algorithmic bytes of the oriented clique walk
+ *
+ * Pinning (what checks each function) is listed in oracle/README and in
+ * tests/test_oracle.py.
+ */
+#ifndef GAT_ORACLE_H
+#define GAT_ORACLE_H
+#include <stdint.h>
+
+#include "../include/gat.h" /* gat_schedule_data layout only */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_schedule(const char *adj, gat_schedule_data *out);
+int64_t orc_automorphisms(const char *adj);
+void orc_graph_cnt(int k, int64_t *even, int64_t *odd);
+
+uint64_t orc_naive_count(int32_t n, const int64_t *off, const int32_t *adj, const char *pattern);
+uint64_t orc_labeled_count(int32_t n, const int64_t *off, const int32_t *adj, const char *pattern);
+/* Reference engine semantics for a schedule built by orc_schedule. Returns
+ * the raw long long the reference would print, as uint64 bits. *ok = 0 when
+ * the schedule makes the reference read an undefined buffer. */
+uint64_t orc_reference_count(int32_t n, const int64_t *off, const int32_t *adj, const gat_schedule_data *s,
+                             int *ok);
+/* k-clique walk on a graph whose ids are already in descending-degree
+ * order: returns #K_k and the algorithmic bytes in *walg. */
+uint64_t orc_clique_walg(int32_t n, const int64_t *off, const int32_t *adj, int k, uint64_t *walg);
+/* Relabel to descending degree (ties: ascending id). Output arrays must
+ * hold n+1 offsets and off[n] entries. */
+void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off_out, int32_t *adj_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""GPU parity: counts from the HIP engine (through the C ABI) against the CPU
+oracle on the same seeded graphs, bit for bit.
+
+unique   == oracle labeled count / |Aut|   (backtracking, schedule-independent)
+reference== oracle restatement of the reference engine for the same schedule
+labeled  == oracle labeled count            (= naive_pm.cu's quantity)
+"""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from patterns import C5, HOUSE, K4, K5, P3, TRIANGLE, clique, connected_patterns
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [
+    {},  # engine defaults
+    {"slot_capacity": 8, "queue_chunk": 1},  # tiny LDS buffers: most tasks take the HBM overflow launch
+    {"queue_chunk": 64, "intersect_strategy": 1, "block_waves": 2, "waves_per_cu": 4},
+    {"intersect_strategy": 2, "search_linear_threshold": 16, "gallop_ratio": 2},
+]
+
+
+def graphs(gat):
+    out = {
+        "er200": gat.Graph.erdos_renyi(200, 0.08, 7),
+        "rmat8": gat.Graph.rmat(8, 6, 0.57, 0.19, 0.19, 3),
+        "cl300": gat.Graph.chung_lu(300, 6.0, 2.1, 40, 5),
+    }
+    return out
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small_graphs(gat, gpu):
+    return graphs(gat)
+
+
+@pytest.mark.parametrize("pattern", [TRIANGLE, K4, K5, HOUSE, C5, P3])
+@pytest.mark.parametrize("gname", ["er200", "rmat8", "cl300"])
+def test_unique_matches_oracle(gat, orc, small_graphs, pattern, gname):
+    g = small_graphs[gname]
+    off, adj = g.csr()
+    want = orc.unique_count(off, adj, pattern)
+    plan = gat.Plan(g, [pattern], "unique")
+    for cfg in CONFIGS:
+        say("unique", gname, pattern, cfg)
+        got, st = plan.count(cfg)
+        assert got[0] == want, (gname, pattern, cfg, got[0], want)
+
+
+@pytest.mark.parametrize("pattern", [TRIANGLE, K4, K5, HOUSE, C5])
+@pytest.mark.parametrize("gname", ["er200", "rmat8", "cl300"])
+def test_reference_matches_oracle(gat, orc, small_graphs, pattern, gname):
+    from graphautotune_amd._lib import ScheduleData
+    g = small_graphs[gname]
+    off, adj = g.csr()
+    sched = orc.schedule(pattern, ScheduleData)
+    want, ok = orc.reference_count(off, adj, sched)
+    assert ok
+    plan = gat.Plan(g, [pattern], "reference")
+    for cfg in CONFIGS:
+        say("reference", gname, pattern, cfg)
+        got, _ = plan.count(cfg)
+        assert got[0] == want, (gname, pattern, cfg, got[0], want)
+
+
+def test_labeled_matches_naive(gat, orc, gpu):
+    g = gat.Graph.erdos_renyi(40, 0.25, 11)
+    off, adj = g.csr()
+    for p in (TRIANGLE, K4, HOUSE):
+        say("labeled", p)
+        want = orc.naive_count(off, adj, p)  # target/naive_pm.cu semantics
+        got, _ = gat.Plan(g, [p], "labeled").count()
+        assert got[0] == want
+
+
+def test_reference_is_k_times_unique_for_cliques(gat, gpu):
+    g = gat.Graph.rmat(10, 8, 0.57, 0.19, 0.19, 9)
+    for k in (3, 4, 5):
+        say("clique", k)
+        u, _ = gat.Plan(g, [clique(k)], "unique").count()
+        r, _ = gat.Plan(g, [clique(k)], "reference").count()
+        assert r[0] == k * u[0]
+
+
+def test_all_connected_5_vertex_patterns_unique(gat, orc, gpu):
+    g = gat.Graph.erdos_renyi(120, 0.12, 21)
+    off, adj = g.csr()
+    pats = connected_patterns(5)
+    want = [orc.unique_count(off, adj, p) for p in pats]
+    say("batched 21x5v")
+    got, _ = gat.Plan(g, pats, "unique").count()
+    assert got == want
+    for p, w in zip(pats, want):  # plain single-pattern plans agree too
+        got1, _ = gat.Plan(g, [p], "unique").count({"slot_capacity": 16})
+        assert got1[0] == w, p
+
+
+def test_random_6_vertex_patterns_unique(gat, orc, gpu):
+    rng = np.random.default_rng(6)
+    pats6 = connected_patterns(6)
+    pick = [pats6[i] for i in rng.choice(len(pats6), 12, replace=False)]
+    g = gat.Graph.erdos_renyi(60, 0.2, 5)
+    off, adj = g.csr()
+    want = [orc.unique_count(off, adj, p) for p in pick]
+    say("batched 12x6v", pick)
+    got, _ = gat.Plan(g, pick, "unique").count({"slot_capacity": 32})
+    assert got == want
+
+
+def test_golden_intersection_15(gat, gpu):
+    # tests/test_vertex_set.cu:44-48,76: |{1..30} ∩ {2,4,..,60}| = 15. As a
+    # graph: a ~ 1..30, b ~ even 2..60, edge a-b; triangles = common
+    # neighbours of a and b.
+    a, b = 61, 62
+    src = [a] * 30 + [b] * 30 + [a]
+    dst = list(range(1, 31)) + [2 * i for i in range(1, 31)] + [b]
+    g = gat.Graph.from_edges(63, src, dst)
+    got, _ = gat.Plan(g, [TRIANGLE], "unique").count()
+    assert got[0] == 15
+
+
+def test_edge_cases(gat, gpu):
+    empty = gat.Graph.from_edges(5, [], [])
+    assert gat.Plan(empty, [TRIANGLE, K4], "unique").count()[0] == [0, 0]
+    star = gat.Graph.from_edges(10, [0] * 9, list(range(1, 10)))
+    assert gat.Plan(star, [TRIANGLE], "unique").count()[0] == [0]
+    kk = gat.Graph.from_edges(9, *zip(*[(i, j) for i in range(9) for j in range(i + 1, 9)]))
+    from math import comb
+    got, _ = gat.Plan(kk, [TRIANGLE, K4, K5, C5, HOUSE], "unique").count()
+    assert got == [comb(9, 3), comb(9, 4), comb(9, 5), 12 * comb(9, 5), 60 * comb(9, 5)]
+    # complete bipartite K_{4,5}: no odd cycles
+    bip = gat.Graph.from_edges(9, *zip(*[(i, j) for i in range(4) for j in range(4, 9)]))
+    got, _ = gat.Plan(bip, [TRIANGLE, K4, C5, HOUSE], "unique").count()
+    assert got == [0, 0, 0, 0]
+
+
+def test_graph_replay_matches_eager(gat, gpu):
+    g = gat.Graph.rmat(9, 8, 0.57, 0.19, 0.19, 3)
+    plan = gat.Plan(g, [K4, HOUSE], "unique")
+    eager, _ = plan.count()
+    for _ in range(3):
+        got, _ = plan.count(graph_replays=3)
+        assert got == eager
+        again, _ = plan.count()  # eager work between replays
+        assert again == eager
+
+
+def test_walg_matches_oracle(gat, orc, gpu):
+    g = gat.Graph.rmat(9, 8, 0.57, 0.19, 0.19, 3)
+    off, adj = g.csr()
+    ro, ra = orc.relabel(off, adj)
+    for k in (3, 4, 5):
+        want_c, want_w = orc.clique_walg(ro, ra, k)
+        got, st = gat.Plan(g, [clique(k)], "unique").count({}, account=True)
+        assert got[0] == want_c
+        assert st["alg_bytes"] == want_w, (k, st["alg_bytes"], want_w)

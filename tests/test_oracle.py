@@ -1,0 +1,251 @@
+"""Pins for the CPU oracle itself: every oracle function is checked against
+closed forms, brute force or an independent derivation, chosen so that a
+plausible mistake (a dropped term, a wrong sign, a transposed index) fails.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from patterns import C5, HOUSE, K4, K5, P3, TRIANGLE, clique, connected_patterns
+
+
+def csr_of_edges(n, edges):
+    adj = [[] for _ in range(n)]
+    for u, v in edges:
+        if u != v:
+            adj[u].append(v)
+            adj[v].append(u)
+    off = [0]
+    flat = []
+    for r in adj:
+        r = sorted(set(r))
+        flat += r
+        off.append(len(flat))
+    return np.array(off, dtype=np.int64), np.array(flat, dtype=np.int32)
+
+
+def complete(m):
+    return csr_of_edges(m, [(i, j) for i in range(m) for j in range(i + 1, m)])
+
+
+def random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    return csr_of_edges(n, [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < p])
+
+
+def to_dense(off, adj):
+    n = len(off) - 1
+    A = np.zeros((n, n), dtype=np.int64)
+    for v in range(n):
+        A[v, adj[off[v]:off[v + 1]]] = 1
+    return A
+
+
+def brute_labeled(off, adj, pattern):
+    """All injective maps, by definition (tiny graphs only)."""
+    n = len(off) - 1
+    A = to_dense(off, adj)
+    k = int(round(len(pattern) ** 0.5))
+    pe = [(i, j) for i in range(k) for j in range(i + 1, k) if pattern[i * k + j] == "1" or pattern[j * k + i] == "1"]
+    return sum(all(A[m[i], m[j]] for i, j in pe) for m in itertools.permutations(range(n), k))
+
+
+# ---- schedule.hpp:149-170 (connected labelled graphs by edge parity) ----
+def test_graph_cnt_closed_forms(orc):
+    connected_labelled = [1, 1, 4, 38, 728]  # OEIS A001187, k = 1..5
+    for k in range(1, 6):
+        e, o = orc.graph_cnt(k)
+        assert e + o == connected_labelled[k - 1]
+        assert e - o == (-1) ** (k - 1) * math.factorial(k - 1)  # Moebius of the partition lattice
+
+
+# ---- schedule.hpp:173-191 ----
+def test_automorphisms_by_orbit_counting(orc):
+    """|Aut| = n! / #distinct relabellings (orbit-stabiliser), an independent
+    derivation from the set of images rather than the stabiliser."""
+    for n in (3, 4, 5):
+        for p in connected_patterns(n):
+            images = set()
+            for perm in itertools.permutations(range(n)):
+                s = ["0"] * (n * n)
+                for i in range(n):
+                    for j in range(n):
+                        if p[i * n + j] == "1":
+                            s[perm[i] * n + perm[j]] = "1"
+                images.add("".join(s))
+            assert orc.automorphisms(p) * len(images) == math.factorial(n)
+    assert orc.automorphisms(C5) == 10
+    assert orc.automorphisms(HOUSE) == 2
+    assert orc.automorphisms(clique(5)) == 120
+
+
+# ---- labelled embeddings: closed forms and brute force ----
+def test_labeled_closed_forms(orc):
+    for m in (5, 7, 9):
+        off, adj = complete(m)
+        for p in (TRIANGLE, K4, K5, HOUSE, C5):
+            k = int(round(len(p) ** 0.5))
+            assert orc.labeled_count(off, adj, p) == math.perm(m, k)
+        assert orc.unique_count(off, adj, C5) == 12 * math.comb(m, 5)
+        assert orc.unique_count(off, adj, HOUSE) == 60 * math.comb(m, 5)
+    # complete bipartite: no odd cycle, C4 = C(a,2) C(b,2)
+    a, b = 4, 5
+    off, adj = csr_of_edges(a + b, [(i, a + j) for i in range(a) for j in range(b)])
+    for p in (TRIANGLE, K4, C5, HOUSE):
+        assert orc.labeled_count(off, adj, p) == 0
+    c4 = "0101101001011010"
+    assert orc.unique_count(off, adj, c4) == math.comb(a, 2) * math.comb(b, 2)
+
+
+def test_triangles_trace(orc):
+    off, adj = random_graph(40, 0.2, 3)
+    A = to_dense(off, adj)
+    assert orc.unique_count(off, adj, TRIANGLE) == np.trace(A @ A @ A) // 6
+
+
+def test_labeled_equals_brute_force_and_naive(orc):
+    for seed in range(3):
+        off, adj = random_graph(8, 0.5, seed)
+        for p in (TRIANGLE, K4, HOUSE, C5, "0110101111010110"):
+            want = brute_labeled(off, adj, p)
+            assert orc.labeled_count(off, adj, p) == want
+            assert orc.naive_count(off, adj, p) == want  # naive_pm.cu restatement
+
+
+# ---- reference engine semantics ----
+def test_reference_cliques_are_k_times_unique(orc):
+    from graphautotune_amd._lib import ScheduleData
+    off, adj = random_graph(30, 0.4, 5)
+    for k in (3, 4, 5):
+        u = orc.unique_count(off, adj, clique(k))
+        r, ok = orc.reference_count(off, adj, orc.schedule(clique(k), ScheduleData))
+        assert ok and r == k * u
+
+
+def _reference_brute(off, adj, s):
+    """The reference engine's value written from its definition by a separate
+    enumeration over all tuples (v_0..v_{b-1}) (tiny graphs only):
+      * dup(l): v_l equals an earlier vertex; every prefix set computed at a
+        dup level is emptied (engine_kernel.cuh:243-258), so a prefix set is
+        empty when any level of its chain is a dup, else the intersection of
+        the neighbourhoods of its vertices;
+      * v_l must lie in its loop set and below v_f for each filtered
+        restriction (f, l) (engine_kernel.cuh:290-338);
+      * IEP factors: size-only prefixes exclude the vertices chosen before
+        their last level (array_vertex_set.cuh:248-275), a size-only
+        one-vertex prefix is the plain degree, other prefixes subtract every
+        chosen vertex found in them (array_vertex_set.cuh:318-331)."""
+    n = len(off) - 1
+    N = [set(adj[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    b = s.basic_vertexes
+    pref = [list(s.prefix_data[i][:s.prefix_depth[i]]) for i in range(s.total_prefix_num)]
+    so = list(s.prefixs_size_only[:s.total_prefix_num])
+    restr = list(zip(s.restrictions[:s.total_restrictions_num], s.restriction_second[:s.total_restrictions_num]))
+    total = 0
+    for tup in itertools.product(range(n), repeat=b):
+        dup = [any(tup[l] == tup[i] for i in range(l)) for l in range(b)]
+
+        def setof(d):
+            if any(dup[j] for j in d):
+                return set()
+            return set.intersection(*[N[tup[j]] for j in d])
+
+        ok = True
+        for l in range(1, b):
+            if tup[l] not in setof(pref[s.loop_set_prefix_id[l]]):
+                ok = False
+                break
+            if any(t == l and not tup[l] < tup[f] for f, t in restr):
+                ok = False
+                break
+        if not ok:
+            continue
+        ans = []
+        for pid in s.iep_vertex_id[:s.iep_prefix_num]:
+            d = pref[pid]
+            S = setof(d)
+            if not S:
+                ans.append(0)
+            elif so[pid] and len(d) > 1:
+                ans.append(len(S - {tup[i] for i in range(d[-1])}))
+            elif so[pid]:
+                ans.append(len(S))
+            else:
+                ans.append(len(S) - sum(tup[i] in S for i in range(b)))
+        val, last, local = 1, -1, 0
+        for g in range(s.subgroups_num):
+            a = ans[s.iep_ans_pos[g]]
+            val = a if g == last + 1 else val * a
+            if s.iep_flag[g]:
+                last = g
+                local += val * s.iep_coef[g]
+        total += local
+    return total % (1 << 64)
+
+
+def test_reference_matches_definition_brute_force(orc):
+    from graphautotune_amd._lib import ScheduleData
+    for seed in range(2):
+        off, adj = random_graph(9, 0.5, 10 + seed)
+        for p in (TRIANGLE, K4, HOUSE, C5, P3):
+            s = orc.schedule(p, ScheduleData)
+            got, ok = orc.reference_count(off, adj, s)
+            assert ok
+            assert got == _reference_brute(off, adj, s), p
+
+
+def test_house_reference_equals_unique(orc):
+    """For the house the emptied duplicate branches contribute 0 and all IEP
+    prefixes end at the last basic vertex, so reference == unique."""
+    from graphautotune_amd._lib import ScheduleData
+    off, adj = random_graph(25, 0.35, 4)
+    r, ok = orc.reference_count(off, adj, orc.schedule(HOUSE, ScheduleData))
+    assert ok and r == orc.unique_count(off, adj, HOUSE)
+
+
+def test_golden_intersection_15(orc):
+    # tests/test_vertex_set.cu:44-48,76 as a graph (see test_gpu_parity)
+    a, b = 61, 62
+    edges = [(a, i) for i in range(1, 31)] + [(b, 2 * i) for i in range(1, 31)] + [(a, b)]
+    off, adj = csr_of_edges(63, edges)
+    assert orc.unique_count(off, adj, TRIANGLE) == 15
+
+
+def test_relabel_and_clique_walg(orc):
+    off, adj = random_graph(60, 0.15, 2)
+    ro, ra = orc.relabel(off, adj)
+    deg = np.diff(ro)
+    assert np.all(deg[:-1] >= deg[1:])  # descending degree order
+    assert sorted(np.diff(off).tolist()) == sorted(deg.tolist())
+    for k in (3, 4, 5):
+        c, w = orc.clique_walg(ro, ra, k)
+        assert c == orc.unique_count(off, adj, clique(k))
+        # W_alg by its definition, recomputed independently in Python
+        N = [ra[ro[v]:ro[v + 1]] for v in range(len(ro) - 1)]
+        S = lambda v: [x for x in N[v] if x < v]  # noqa: E731
+        tot = 0
+
+        def rec(Sprev, d, chosen):
+            nonlocal tot
+            for v in Sprev:
+                A = S(v)
+                B = [x for x in Sprev if x < v]
+                out = sorted(set(A) & set(B))
+                if d == k - 2:
+                    tot += 4 * (len(A) + len(B)) + 16
+                else:
+                    tot += 4 * (len(A) + len(B)) + 4 * len(out) + 16
+                    rec(out, d + 1, chosen + [v])
+
+        for v0 in range(len(ro) - 1):
+            for v1 in S(v0):
+                A, B = S(v1), [x for x in S(v0) if x < v1]
+                out = sorted(set(A) & set(B))
+                if k == 3:
+                    tot += 4 * (len(A) + len(B)) + 32
+                else:
+                    tot += 4 * (len(A) + len(B)) + 4 * len(out) + 32
+                    rec(out, 2, [v0, v1])
+        assert w == tot
