@@ -1,0 +1,210 @@
+"""Benchmark: exact K4 + K5 counts on a seeded R-MAT graph (scale 20, edge
+factor 16, (a,b,c,d) = (.57,.19,.19,.05), seed 20) on one MI355X.
+
+A step is one batched count of both patterns (one persistent launch plus the
+HBM-overflow launch). `value` is undirected edges / median device time of a
+batched step (inputs resident in HBM). The plain single-pattern engine is
+timed beside it; `batched_vs_plain` = (t_K4 + t_K5) / t_batched at identical
+counts. `roofline` is for the main engine kernel: W_alg bytes of one launch
+(instrumented pass) / its hipEvent-timed duration, against 8 TB/s.
+`cpu_baseline` times the CPU oracle's oriented clique walk on a bounded
+sample of first vertices and scales it by algorithmic bytes.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "exact-count throughput (edges/s), batched+tuned vs plain, R-MAT s20 4/5-clique"
+K4 = "0111101111011110"
+K5 = "0111110111110111110111110"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tuned_config(scale):
+    """Kernel config picked by the autotuner for this workload, if recorded."""
+    path = os.path.join(ROOT, "graphautotune_amd", "tuning", "tuned_configs.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            d = json.load(f)
+        key = "rmat%d-k4k5" % scale
+        if key in d:
+            return d[key]["config"], key
+    return {}, None
+
+
+def timed(plan, cfg, steps):
+    secs, mains, walls, counts = [], [], [], None
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        c, st = plan.count(cfg)
+        walls.append(time.perf_counter() - t0)
+        secs.append(st["seconds"])
+        mains.append(st["main_kernel_ms"])
+        if counts is None:
+            counts = c
+        elif c != counts:
+            raise AssertionError("counts changed between steps: %s vs %s" % (c, counts))
+    return counts, secs, mains, walls, st
+
+
+def cpu_baseline(g, total_walg, budget_s):
+    """Oracle clique walk (single thread) on every stride-th first vertex."""
+    from oracle import api as orc
+    off, adj = g.csr()
+    t0 = time.perf_counter()
+    ro, ra = orc.relabel(off, adj)
+    log("cpu: relabel %.1fs" % (time.perf_counter() - t0))
+    n = len(ro) - 1
+    stride = 1 << 16
+    while True:
+        t0 = time.perf_counter()
+        w = 0
+        for k in (4, 5):
+            w += orc.clique_walg_sample(ro, ra, k, stride, 0)[1]
+        dt = time.perf_counter() - t0
+        log("cpu: stride %d took %.2fs (W %.3g B)" % (stride, dt, w))
+        if dt >= budget_s / 4 or stride == 1:
+            break
+        stride = max(1, int(stride / min(64.0, (budget_s / 2) / max(dt, 0.01))))
+    est = dt * total_walg / max(w, 1)
+    return {
+        "value": g.undirected_edges() / est,
+        "unit": "edges/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle oriented-clique walk (K4 then K5) over first vertices v0 = 0, %d, 2*%d, ... "
+                  "(%d of %d vertices, %.1f s, %.3g of %.3g algorithmic bytes); time scaled by bytes"
+                  % (stride, stride, (n + stride - 1) // stride, n, dt, w, total_walg),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", type=str, default=None, help="JSON kernel config for the batched run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import graphautotune_amd as gat
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        sync = lambda: None  # noqa: E731
+
+    t0 = time.perf_counter()
+    g = gat.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, 20)
+    E = g.undirected_edges()
+    log("graph: R-MAT s%d: %d vertices, %d edges, max degree %d (%.1fs)"
+        % (args.scale, g.v_cnt(), E, g.max_degree(), time.perf_counter() - t0))
+    cfg, tuned_key = tuned_config(args.scale)
+    if args.config:
+        cfg = json.loads(args.config)
+    batched = gat.Plan(g, [K4, K5], "unique")
+    plain4 = gat.Plan(g, [K4], "unique")
+    plain5 = gat.Plan(g, [K5], "unique")
+
+    for _ in range(args.warmup):
+        batched.count(cfg)
+        plain4.count({})
+        plain5.count({})
+    # instrumented pass: algorithmic bytes (not timed)
+    acc_counts, acc = batched.count(cfg, account=True)
+    walg = acc["alg_bytes"]
+    walg_main = acc["alg_bytes_main"]
+    _, acc4 = plain4.count({}, account=True)
+    _, acc5 = plain5.count({}, account=True)
+    log("W_alg batched %.4g B, plain %.4g + %.4g B" % (walg, acc4["alg_bytes"], acc5["alg_bytes"]))
+
+    sync()
+    tw0 = time.perf_counter()
+    counts, secs, mains, walls, st = timed(batched, cfg, args.steps)
+    sync()
+    wall = time.perf_counter() - tw0
+    c4, s4, _, _, _ = timed(plain4, {}, args.steps)
+    c5, s5, _, _, _ = timed(plain5, {}, args.steps)
+    if counts != acc_counts or counts != [c4[0], c5[0]]:
+        raise AssertionError("batched %s, instrumented %s and plain %s counts differ" % (counts, acc_counts, c4 + c5))
+    t_b = statistics.median(secs)
+    t_plain = statistics.median(s4) + statistics.median(s5)
+    main_ms = statistics.median(mains)
+    achieved = walg_main / (main_ms * 1e-3) / 1e9
+    log("counts K4 %d K5 %d | batched %.3f ms (main %.3f ms), plain %.3f ms | overflow tasks %d"
+        % (counts[0], counts[1], t_b * 1e3, main_ms, t_plain * 1e3, st["overflow_tasks"]))
+
+    kernel_name = "gatk::clique_kernel" if cfg.get("vertex_store", 0) != 1 else "gatk::engine_kernel<false>"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_rmat%d_k4k5.json" % args.scale)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": E / t_b,
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": "R-MAT scale %d, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 20; exact unique "
+                        "K4 + K5 counts in one batched launch" % args.scale,
+            "vertices": g.v_cnt(),
+            "undirected_edges": E,
+            "patterns": {"K4": K4, "K5": K5},
+            "kernel_config": cfg,
+            "tuned_from": tuned_key,
+        },
+        "counts": {"K4": counts[0], "K5": counts[1]},
+        "reference_mode_counts": {"K4": 4 * counts[0], "K5": 5 * counts[1]},
+        "device_ms_batched": t_b * 1e3,
+        "device_ms_plain": {"K4": statistics.median(s4) * 1e3, "K5": statistics.median(s5) * 1e3},
+        "plain_value": E / t_plain,
+        "batched_vs_plain": t_plain / t_b,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "alg_bytes_per_launch": walg_main,
+            "kernel": kernel_name,
+            "kernel_ms": main_ms,
+            "step_alg_bytes": walg,
+            "step_achieved": walg / t_b / 1e9,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(g, acc4["alg_bytes"] + acc5["alg_bytes"], args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -76,11 +76,12 @@ class KernelConfig(C.Structure):
         ("gallop_ratio", C.c_int32),
         ("search_linear_threshold", C.c_int32),
         ("max_regs", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("vertex_store", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
     KNOBS = ("block_waves", "waves_per_cu", "queue_chunk", "slot_capacity", "leaf_mode",
-             "intersect_strategy", "gallop_ratio", "search_linear_threshold", "max_regs")
+             "intersect_strategy", "gallop_ratio", "search_linear_threshold", "max_regs", "vertex_store")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.KNOBS}
@@ -99,6 +100,7 @@ class CountStats(C.Structure):
         ("main_kernel_ms", C.c_double),
         ("overflow_tasks", C.c_uint64),
         ("alg_bytes", C.c_uint64),
+        ("alg_bytes_main", C.c_uint64),
         ("tasks", C.c_uint64),
         ("grid_blocks", C.c_int32),
         ("block_threads", C.c_int32),

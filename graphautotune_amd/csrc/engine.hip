@@ -608,6 +608,366 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
     if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
 }
 
+// ---------------------------------------------------------------------------
+// Bitmap vertex store for clique plans (the reference declares
+// VertexStoreType::Bitmap in include/configs/vertex_set_config.hpp:3-6 but
+// never implements it). On the degree-oriented graph every k-clique is a
+// chain v0 > v1 > ... with v_{i+1} in N+(v_i). Per oriented edge (v0, v1)
+// the wave materialises S = N+(v0) ∩ N+(v1) (sorted, in LDS) and the
+// oriented adjacency of the subgraph induced on S as bit rows
+//   row_i = { j < i : S[j] in N+(S[i]) }.
+// Then #K3 = |S|, #K4 = sum_i popc(row_i), #K5 = sum_i sum_{j in row_i}
+// popc(row_i & row_j), #K6 one level deeper. Rows are built by one
+// flattened sweep over the lists N+(S[i]) (binary search in S, or the
+// reverse for skewed pairs), so the two deepest levels become LDS bit
+// operations instead of list intersections. Edges whose S outgrows the LDS
+// rows are queued for the array-store engine.
+constexpr int CMAXW = 8;  // row words: |S| <= 256
+
+struct CArgs {
+    const long long *off;
+    const int *adj;
+    const int *split;
+    int n;
+    long long m;
+    unsigned long long *counts;
+    int pat_of_k[8];
+    int kmax;
+    unsigned long long *queue;
+    long long *ovf;
+    unsigned long long *ovf_n;
+    long long ovf_cap;
+    unsigned long long *bytes;
+    unsigned int *err;
+    int account;
+    int cap_s;  // |S| limit for bit rows (<= 32 * CMAXW)
+    int cap0;   // N+(v0) staging entries
+    int chunk;
+    int lin_thr;
+    int gallop_ratio;
+};
+
+struct CliqueWave {
+    unsigned long long acc[8];
+    unsigned long long bytes;
+    int s_len;
+    int pad;
+};
+
+__device__ __forceinline__ int clique_wave_bytes(int cap0, int cap_s) {
+    const int w = (cap_s + 31) / 32;
+    return ((int)sizeof(CliqueWave) + 4 * cap0 + 4 * cap_s + 4 * cap_s * w + 15) / 16 * 16;
+}
+
+// S = A ∩ B for sorted lists; writes at most cap entries to out (LDS),
+// returns the full size. Streams the shorter list, searches the longer.
+template <class AP, class BP>
+__device__ __forceinline__ int clique_isect(AP A, int alen, BP B, int blen, int *out, int cap, int lin_thr,
+                                            int ln) {
+    int total = 0;
+    const bool sa = alen <= blen;
+    const int slen = sa ? alen : blen;
+    for (int base = 0; base < slen; base += W) {
+        const int i = base + ln;
+        bool hit = false;
+        int x = 0;
+        if (i < slen) {
+            x = sa ? A[i] : B[i];
+            hit = sa ? has(B, blen, x, lin_thr) : has(A, alen, x, lin_thr);
+        }
+        const unsigned long long mk = ballot(hit);
+        const int pos = total + mbcnt(mk);
+        if (hit && pos < cap) out[pos] = x;
+        total += __popcll(mk);
+    }
+    return total;
+}
+
+__global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wid = threadIdx.x / W, ln = threadIdx.x % W;
+    unsigned char *base = smem + (size_t)wid * wave_bytes;
+    CliqueWave *S = reinterpret_cast<CliqueWave *>(base);
+    int *n0 = reinterpret_cast<int *>(base + sizeof(CliqueWave));
+    int *sv = n0 + a.cap0;
+    unsigned *rows = reinterpret_cast<unsigned *>(sv + a.cap_s);
+    if (ln < 8) S->acc[ln] = 0;
+    if (ln == 0) S->bytes = 0;
+    wsync();
+    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;  // per lane
+    for (;;) {
+        unsigned long long r = 0;
+        if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
+        const long long e0 = uni64((long long)__shfl(r, 0));
+        if (e0 >= a.m) break;
+        const long long e1 = e0 + a.chunk < a.m ? e0 + a.chunk : a.m;
+        long long e = e0;
+        while (e < e1) {
+            int lo = 0, hi = a.n;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (a.off[mid] <= e) lo = mid; else hi = mid;
+            }
+            const int v0 = uni(lo);
+            const long long rb = a.off[v0], re = a.off[v0 + 1];
+            const long long se = e1 < re ? e1 : re;
+            const long long oe = rb + a.split[v0];  // out-edges end here
+            const long long hi_e = se < oe ? se : oe;
+            if (e < hi_e) {
+                const int need = (int)(hi_e - rb);  // N+(v0)[0, need) is all these edges look at
+                const bool staged = need <= a.cap0;
+                if (staged) {
+                    for (int k = ln; k < need; k += W) n0[k] = a.adj[rb + k];
+                    wsync();
+                }
+                for (long long ee = e; ee < hi_e; ee++) {
+                    const int i = (int)(ee - rb);
+                    const int v1 = uni(a.adj[ee]);
+                    const long long ab = a.off[v1];
+                    const int alen = a.split[v1];
+                    const int cap = a.kmax >= 4 ? a.cap_s : 0;
+                    int s = staged ? clique_isect(a.adj + ab, alen, n0, i, sv, cap, a.lin_thr, ln)
+                                   : clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, cap, a.lin_thr, ln);
+                    s = uni(s);
+                    if (a.account && ln == 0 && !(a.kmax >= 4 && s > a.cap_s))
+                        bytes += 4ull * (alen + i) + (a.kmax >= 4 ? 4ull * s : 0ull) + 32ull;
+                    if (a.kmax >= 4 && s > a.cap_s) {  // rows do not fit: the HBM-row kernel takes this edge
+                        if (ln == 0) {
+                            const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
+                            if ((long long)idx < a.ovf_cap) a.ovf[idx] = ee; else atomicOr(a.err, 4u);
+                        }
+                        continue;
+                    }
+                    if (ln == 0) acc3 += (unsigned long long)s;
+                    if (a.kmax < 4 || s < 2) continue;
+                    wsync();
+                    // ---- bit rows of the subgraph induced on S ----
+                    const int w = (s + 31) >> 5;
+                    for (int k = ln; k < s * w; k += W) rows[k] = 0u;
+                    wsync();
+                    for (int cb = 0; cb < s; cb += W) {
+                        const int ci = cb + ln;
+                        const bool valid = ci < s;
+                        const int x = valid ? sv[ci] : 0;
+                        const long long xb = valid ? a.off[x] : 0;
+                        const int xl = valid ? a.split[x] : 0;  // N+(x): all below x
+                        // stream N+(x) and search S[0,ci), or the reverse when skewed
+                        const bool sx = (long long)xl <= (long long)a.gallop_ratio * (long long)ci;
+                        const int mylen = valid ? (sx ? xl : ci) : 0;
+                        if (a.account && valid) bytes += 4ull * (xl + ci) + 16ull;
+                        int T = 0;
+                        const int myoff = excl_scan(mylen, &T);
+                        for (int k0 = 0; k0 < T; k0 += W) {
+                            const int j = k0 + ln;
+                            int so = 0;
+#pragma unroll
+                            for (int step = 32; step >= 1; step >>= 1)
+                                if (bperm(myoff, so + step) <= j && so + step < W) so += step;
+                            // every lane takes part in the cross-lane reads: a
+                            // segment's owner lane may itself have no element here
+                            const int s_off = bperm(myoff, so);
+                            const int s_sx = bperm((int)sx, so);
+                            const long long s_xb = bperm64(xb, so);
+                            const int s_xl = bperm(xl, so);
+                            if (j < T) {
+                                const int s_ci = cb + so;
+                                const int idx = j - s_off;
+                                int pos = -1;
+                                if (s_sx) {
+                                    const int y = a.adj[s_xb + idx];
+                                    const int p = lbound(sv, s_ci, y);
+                                    if (p < s_ci && sv[p] == y) pos = p;
+                                } else {
+                                    if (has(a.adj + s_xb, s_xl, sv[idx], a.lin_thr)) pos = idx;
+                                }
+                                if (pos >= 0) atomicOr(&rows[s_ci * w + (pos >> 5)], 1u << (pos & 31));
+                            }
+                        }
+                    }
+                    wsync();
+                    // ---- counts from the rows ----
+                    for (int k = ln; k < s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
+                    if (a.account && ln == 0) bytes += 4ull * s * w;
+                    if (a.kmax >= 5) {
+                        for (int ib = 0; ib < s; ib += W) {
+                            const int ii = ib + ln;
+                            if (ii >= s) continue;
+                            unsigned ri[CMAXW];
+#pragma unroll
+                            for (int q = 0; q < CMAXW; q++) ri[q] = q < w ? rows[ii * w + q] : 0u;
+                            for (int q = 0; q < CMAXW; q++) {
+                                if (q >= w) break;
+                                unsigned bits = ri[q];
+                                while (bits) {
+                                    const int jj = q * 32 + __ffs(bits) - 1;
+                                    bits &= bits - 1;
+                                    unsigned rm[CMAXW];
+                                    unsigned long long c5 = 0;
+#pragma unroll
+                                    for (int t = 0; t < CMAXW; t++) {
+                                        rm[t] = t < w ? (ri[t] & rows[jj * w + t]) : 0u;
+                                        c5 += (unsigned long long)__popc(rm[t]);
+                                    }
+                                    acc5 += c5;
+                                    if (a.account) bytes += 8ull * w;
+                                    if (a.kmax >= 6) {
+                                        for (int t = 0; t < CMAXW; t++) {
+                                            if (t >= w) break;
+                                            unsigned b2 = rm[t];
+                                            while (b2) {
+                                                const int ll = t * 32 + __ffs(b2) - 1;
+                                                b2 &= b2 - 1;
+#pragma unroll
+                                                for (int u = 0; u < CMAXW; u++)
+                                                    if (u < w) acc6 += (unsigned long long)__popc(rm[u] & rows[ll * w + u]);
+                                                if (a.account) bytes += 8ull * w;
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    wsync();
+                }
+            }
+            e = se;
+        }
+    }
+    acc3 = wave_sum64(acc3);
+    acc4 = wave_sum64(acc4);
+    acc5 = wave_sum64(acc5);
+    acc6 = wave_sum64(acc6);
+    bytes = wave_sum64(bytes);
+    if (ln == 0) {
+        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
+        for (int k = 3; k <= 6; k++)
+            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
+        if (a.account && bytes) atomicAdd(a.bytes, bytes);
+    }
+}
+
+// Edges whose S outgrows the LDS rows: one edge per work item, S in LDS
+// (or HBM when the maximum out-degree is too large for LDS), bit rows in a
+// per-wave HBM/L2 scratch area of capacity cap_s x ceil(cap_s/32) words.
+struct CBigArgs {
+    CArgs c;
+    unsigned *grows;  // per-wave rows
+    int *gsv;         // per-wave S when not in LDS
+    int cap_s;        // >= maximum out-degree
+    int sv_lds;       // S staged in LDS
+};
+
+__global__ void __launch_bounds__(64) clique_big_kernel(CBigArgs b) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CArgs &a = b.c;
+    const int ln = threadIdx.x % W;
+    const int wave = blockIdx.x;
+    const int ws = (b.cap_s + 31) / 32;
+    unsigned *rows = b.grows + (long long)wave * b.cap_s * ws;
+    int *sv = b.sv_lds ? reinterpret_cast<int *>(smem) : b.gsv + (long long)wave * b.cap_s;
+    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;
+    for (;;) {
+        unsigned long long r = 0;
+        if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
+        r = (unsigned long long)uni64((long long)__shfl(r, 0));
+        if (r >= *a.ovf_n || (long long)r >= a.ovf_cap) break;
+        const long long ee = a.ovf[r];
+        int lo = 0, hi = a.n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.off[mid] <= ee) lo = mid; else hi = mid;
+        }
+        const int v0 = uni(lo);
+        const long long rb = a.off[v0];
+        const int i = (int)(ee - rb);
+        const int v1 = uni(a.adj[ee]);
+        const long long ab = a.off[v1];
+        const int alen = a.split[v1];
+        int s = uni(clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, b.cap_s, a.lin_thr, ln));
+        if (s > b.cap_s) {  // cannot happen: S lies in N+(v1)
+            if (ln == 0) atomicOr(a.err, 8u);
+            continue;
+        }
+        if (a.account && ln == 0) bytes += 4ull * (alen + i) + 4ull * s + 32ull;
+        if (ln == 0) acc3 += (unsigned long long)s;
+        if (s < 2) continue;
+        const int w = (s + 31) >> 5;
+        for (long long k = ln; k < (long long)s * w; k += W) rows[k] = 0u;
+        __threadfence();
+        wsync();
+        // rows: lanes take elements of S; each streams N+(x) against S[0,ci)
+        // (or the reverse when skewed) -- here the lists are long enough for
+        // a lane-per-row sweep
+        for (int ci = ln; ci < s; ci += W) {
+            const int x = sv[ci];
+            const long long xb = a.off[x];
+            const int xl = a.split[x];
+            if (a.account) bytes += 4ull * (xl + ci) + 16ull;
+            if ((long long)xl <= (long long)a.gallop_ratio * ci) {
+                for (int k = 0; k < xl; k++) {
+                    const int y = a.adj[xb + k];
+                    const int p = lbound(sv, ci, y);
+                    if (p < ci && sv[p] == y) rows[(long long)ci * w + (p >> 5)] |= 1u << (p & 31);
+                }
+            } else {
+                for (int k = 0; k < ci; k++)
+                    if (has(a.adj + xb, xl, sv[k], a.lin_thr)) rows[(long long)ci * w + (k >> 5)] |= 1u << (k & 31);
+            }
+        }
+        __threadfence();
+        wsync();
+        for (long long k = ln; k < (long long)s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
+        if (a.account && ln == 0) bytes += 4ull * s * w;
+        if (a.kmax >= 5) {
+            // lanes take the words of row i; each set bit j adds popc(row_i & row_j)
+            for (int ii = 0; ii < s; ii++) {
+                const unsigned *ri = rows + (long long)ii * w;
+                for (int q = ln; q < w; q += W) {
+                    unsigned bits = ri[q];
+                    while (bits) {
+                        const int jj = q * 32 + __ffs(bits) - 1;
+                        bits &= bits - 1;
+                        const unsigned *rj = rows + (long long)jj * w;
+                        const int wj = (jj >> 5) + 1;  // row j has bits below j only
+                        unsigned long long c5 = 0;
+                        for (int t = 0; t < wj; t++) c5 += (unsigned long long)__popc(ri[t] & rj[t]);
+                        acc5 += c5;
+                        if (a.account) bytes += 8ull * w;
+                        if (a.kmax >= 6) {
+                            for (int t = 0; t < wj; t++) {
+                                unsigned b2 = ri[t] & rj[t];
+                                while (b2) {
+                                    const int ll = t * 32 + __ffs(b2) - 1;
+                                    b2 &= b2 - 1;
+                                    const unsigned *rl = rows + (long long)ll * w;
+                                    const int wl = (ll >> 5) + 1;
+                                    for (int u = 0; u < wl; u++)
+                                        acc6 += (unsigned long long)__popc(ri[u] & rj[u] & rl[u]);
+                                    if (a.account) bytes += 8ull * w;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __threadfence();
+        wsync();
+    }
+    acc3 = wave_sum64(acc3);
+    acc4 = wave_sum64(acc4);
+    acc5 = wave_sum64(acc5);
+    acc6 = wave_sum64(acc6);
+    bytes = wave_sum64(bytes);
+    if (ln == 0) {
+        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
+        for (int k = 3; k <= 6; k++)
+            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
+        if (a.account && bytes) atomicAdd(a.bytes, bytes);
+    }
+}
+
 __global__ void fill_u64(unsigned long long *p, long long n, unsigned long long v) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         p[i] = v;
@@ -700,6 +1060,10 @@ struct PlanDevice {
     size_t gstore_bytes = 0;
     int ovf_grid = 0;
     hipEvent_t ev[4] = {};
+    // heavy-edge bitmap scratch
+    unsigned *grows = nullptr;
+    int *gsv = nullptr;
+    int big_waves = 0, big_cap = 0;
     std::shared_ptr<DeviceGraph> dg;
     // captured graph
     hipGraphExec_t gexec = nullptr;
@@ -713,7 +1077,7 @@ void destroy_plan_device(PlanDevice *d) {
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
-                    (void *)d->err, (void *)d->ovf, (void *)d->gstore})
+                    (void *)d->err, (void *)d->ovf, (void *)d->gstore, (void *)d->grows, (void *)d->gsv})
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
@@ -749,6 +1113,13 @@ struct Launch {
     size_t ovf_lds = 0;
     int ovf_wave_bytes = 0;
     gatk::KArgs ao;
+    bool clique = false;  // bitmap clique path
+    gatk::CArgs ca;
+    size_t clique_lds = 0;
+    int clique_wave_bytes = 0;
+    gatk::CBigArgs cb;
+    dim3 big_grid;
+    size_t big_lds = 0;
 };
 
 static void fill_defaults(gat_kernel_config *c) {
@@ -792,7 +1163,10 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     }
     PlanDevice *d = p->dev;
     const long long chunk = cfg.queue_chunk;
-    const long long ovf_cap = (g.m + chunk - 1) / chunk + 1;
+    const bool clique = p->cp.clique_ks != 0 && cfg.vertex_store != 1;
+    if (cfg.vertex_store == 2 && !clique)
+        GAT_FAIL(GAT_EINFEASIBLE, "bitmap vertex store needs a plan of cliques K3..K6 in unique or labeled mode");
+    const long long ovf_cap = clique ? g.m + 1 : (g.m + chunk - 1) / chunk + 1;
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
         HIPCHECK(hipMalloc(&d->ovf, sizeof(long long) * ovf_cap));
@@ -835,6 +1209,67 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     L->ovf_grid = dim3(d->ovf_grid);
     L->ovf_wave_bytes = int((sizeof(gatk::WaveState) + 15) / 16 * 16);
     L->ovf_lds = L->ovf_wave_bytes;
+    L->clique = clique;
+    if (clique) {
+        gatk::CArgs c{};
+        c.off = a.off;
+        c.adj = a.adj;
+        c.split = a.split;
+        c.n = a.n;
+        c.m = a.m;
+        c.counts = a.counts;
+        for (int k = 0; k < 8; k++) c.pat_of_k[k] = p->cp.clique_pattern[k];
+        c.kmax = 31 - __builtin_clz(p->cp.clique_ks);
+        c.queue = a.queue;
+        c.ovf = a.ovf;
+        c.ovf_n = a.ovf_n;
+        c.ovf_cap = a.ovf_cap;
+        c.bytes = a.bytes;
+        c.err = a.err;
+        c.cap_s = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
+        c.cap0 = std::max(256, std::min(4096, cfg.slot_capacity));
+        c.chunk = cfg.queue_chunk;
+        c.lin_thr = cfg.search_linear_threshold;
+        c.gallop_ratio = cfg.gallop_ratio;
+        const int w = (c.cap_s + 31) / 32;
+        L->clique_wave_bytes = ((int)sizeof(gatk::CliqueWave) + 4 * c.cap0 + 4 * c.cap_s + 4 * c.cap_s * w + 15) / 16 * 16;
+        L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
+        if (L->clique_lds > di.lds_per_block)
+            GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
+        L->ca = c;
+        // heavy edges: rows in HBM sized by the maximum out-degree
+        if (g.max_out < 0) {
+            int32_t mo = 0;
+            for (int32_t v = 0; v < g.n; v++) {
+                const int32_t *b = g.adj.data() + g.off[v], *en = g.adj.data() + g.off[v + 1];
+                mo = std::max(mo, int32_t(std::lower_bound(b, en, v) - b));
+            }
+            g.max_out = mo;
+        }
+        const int cap = std::max(32, g.max_out);
+        const int ws = (cap + 31) / 32;
+        int waves = std::max(64, 8 * di.cus);
+        while (waves > 64 && (double)waves * cap * ws * 4 > 16e9) waves /= 2;
+        if (d->big_cap < cap || d->big_waves < waves) {
+            if (d->grows) HIPCHECK(hipFree(d->grows));
+            if (d->gsv) HIPCHECK(hipFree(d->gsv));
+            d->grows = nullptr;
+            d->gsv = nullptr;
+            HIPCHECK(hipMalloc(&d->grows, (size_t)waves * cap * ws * 4));
+            HIPCHECK(hipMalloc(&d->gsv, (size_t)waves * cap * 4));
+            d->big_cap = cap;
+            d->big_waves = waves;
+        }
+        gatk::CBigArgs cb{};
+        cb.c = c;
+        cb.grows = d->grows;
+        cb.gsv = d->gsv;
+        cb.cap_s = d->big_cap;
+        cb.sv_lds = (size_t)cap * 4 <= std::min<size_t>(48 * 1024, di.lds_per_block) ? 1 : 0;
+        L->cb = cb;
+        L->big_grid = dim3(d->big_waves);
+        L->big_lds = cb.sv_lds ? (size_t)d->big_cap * 4 : 0;
+    }
     return GAT_OK;
 }
 
@@ -848,10 +1283,24 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
     gatk::KArgs a = L.a, ao = L.ao;
     a.account = ao.account = account ? 1 : 0;
+    ao.bytes = d->bytes + 1;  // second launch: its own W_alg counter
     if (events) HIPCHECK(hipEventRecord(d->ev[0], s));
-    hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, a, L.wave_bytes);
+    if (L.clique) {
+        gatk::CArgs c = L.ca;
+        c.account = account ? 1 : 0;
+        hipLaunchKernelGGL(gatk::clique_kernel, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
+    } else {
+        hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, a, L.wave_bytes);
+    }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
-    hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+    if (L.clique) {
+        gatk::CBigArgs cb = L.cb;
+        cb.c.account = account ? 1 : 0;
+        cb.c.bytes = d->bytes + 1;
+        hipLaunchKernelGGL(gatk::clique_big_kernel, L.big_grid, dim3(64), L.big_lds, s, cb);
+    } else {
+        hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+    }
     if (events) HIPCHECK(hipEventRecord(d->ev[2], s));
     HIPCHECK(hipGetLastError());
     return GAT_OK;
@@ -860,11 +1309,11 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
 static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf = 0, bytes = 0;
+    unsigned long long raw[GATP_MAX_TERMS], ovf = 0, bytes[2] = {0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(&ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
-    HIPCHECK(hipMemcpyAsync(&bytes, d->bytes, sizeof(bytes), hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipMemcpyAsync(bytes, d->bytes, sizeof(bytes), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(&err, d->err, sizeof(err), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipStreamSynchronize(d->stream));
     if (err) GAT_FAIL(GAT_ENODEV, "engine reported internal error flags 0x%x", err);
@@ -885,7 +1334,8 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->main_kernel_ms = ms_main;
         }
         st->overflow_tasks = ovf;
-        st->alg_bytes = bytes;
+        st->alg_bytes = bytes[0] + bytes[1];
+        st->alg_bytes_main = bytes[0];
         st->tasks = (uint64_t)p->graph->m;
         st->grid_blocks = (int)L.grid.x;
         st->block_threads = (int)L.block.x;

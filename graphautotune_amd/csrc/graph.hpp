@@ -21,6 +21,7 @@ struct Graph {
     std::vector<int64_t> off;
     std::vector<int32_t> adj;
     int32_t max_deg = 0;
+    int32_t max_out = -1;       // max #neighbours below the vertex (computed on demand)
     std::shared_ptr<void> dev;  // device copy, created by the engine on first use
 
     int32_t deg(int32_t v) const { return int32_t(off[v + 1] - off[v]); }

@@ -396,7 +396,37 @@ int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *ou
         C.terms.push_back(std::move(term));
         cp.patterns.push_back(std::move(info));
     }
-    for (size_t i = 0; i < C.slots.size(); i++) C.resolve((int)i);
+    // A COUNT whose prefix is also materialised as a LIST at the same node is
+    // read off that list (lower bound + exclusions) instead of being
+    // intersected a second time; batched plans hit this when one pattern
+    // counts the set another pattern loops over (K4 + K5).
+    std::vector<int> dead(C.slots.size(), 0);
+    for (size_t i = 0; i < C.slots.size(); i++) {
+        HSlot &cs = C.slots[i];
+        if (cs.kind != GATP_COUNT) continue;
+        int list = -1;
+        for (size_t j = 0; j < C.slots.size(); j++)
+            if (C.slots[j].kind == GATP_LIST && C.slots[j].node == cs.node && C.slots[j].data == cs.data) list = (int)j;
+        if (list < 0) continue;
+        for (HTerm &t : C.terms)
+            for (HVid &v : t.vids)
+                if (v.slot == (int)i && v.mode == GATP_VID_COUNT) {
+                    v.slot = list;
+                    v.mode = GATP_VID_LIST_BOUNDED;
+                    v.bound = cs.T;
+                    v.excl = cs.X;
+                    C.slots[list].uses.push_back(cs.T ? std::make_pair(0, (int)cs.T) : std::make_pair(2, 0));
+                }
+        dead[i] = 1;
+        if (cs.father >= 0) {  // drop the father-use it registered
+            auto &u = C.slots[cs.father].uses;
+            u.erase(std::remove(u.begin(), u.end(), std::make_pair(1, (int)i)), u.end());
+        }
+        auto &ops = C.nodes[cs.node].ops;
+        ops.erase(std::remove(ops.begin(), ops.end(), (int)i), ops.end());
+    }
+    for (size_t i = 0; i < C.slots.size(); i++)
+        if (!dead[i]) C.resolve((int)i);
 
     // ---- emit ----
     if ((int)C.nodes.size() > GATP_MAX_NODES) GAT_FAIL(GAT_ELIMIT, "plan needs %zu nodes", C.nodes.size());
@@ -504,6 +534,26 @@ int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *ou
     d.n_patterns = (int)adj.size();
     d.max_leaf_ops = max_leaf;
     d.max_depth = maxd;
+    // bitmap eligibility (clique-only plans)
+    if (mode != GAT_MODE_REFERENCE) {
+        uint32_t ks = 0;
+        bool all = true;
+        for (size_t i = 0; i < cp.patterns.size(); i++) {
+            const Pattern &p = cp.patterns[i].sched.input;
+            const bool clique = p.edge_count() == p.n * (p.n - 1) / 2;
+            if (!clique || p.n > 6) {
+                all = false;
+                break;
+            }
+            if (cp.clique_pattern[p.n] >= 0) {  // the same clique twice: leave it to the array store
+                all = false;
+                break;
+            }
+            cp.clique_pattern[p.n] = (int)i;
+            ks |= 1u << p.n;
+        }
+        if (all) cp.clique_ks = ks;
+    }
     d.root_slot = -1;
     for (int sid : C.nodes[0].ops)
         if (C.slots[sid].data == std::vector<int>{0}) d.root_slot = sid;

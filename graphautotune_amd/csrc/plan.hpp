@@ -24,6 +24,11 @@ struct CompiledPlan {
     GatPlanDev dev;
     std::vector<PatternInfo> patterns;
     bool relabel = false;  // run on the degree-relabelled graph
+    // Bitmap-eligible plan: every pattern is a clique K_k with 3 <= k <= 6,
+    // counted in unique or labeled mode. Bit k is set for each K_k present;
+    // clique_pattern[k] = index of that pattern (first occurrence).
+    uint32_t clique_ks = 0;
+    int clique_pattern[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
 };
 
 // For unique counting: picks the restriction set to apply and the constant

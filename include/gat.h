@@ -114,7 +114,10 @@ typedef struct gat_kernel_config {
     int32_t search_linear_threshold; /* set_search_type: linear scan below this
                                  size, binary above (array_vertex_set.cuh:145) */
     int32_t max_regs;         /* register cap hint: 0, 64 or 128 VGPRs         */
-    int32_t reserved[3];
+    int32_t vertex_store;     /* 0 auto, 1 sorted arrays only, 2 bitmap rows for
+                                 clique plans (VertexStoreType {Array, Bitmap},
+                                 vertex_set_config.hpp:3-6)                    */
+    int32_t reserved[2];
 } gat_kernel_config;
 
 /* Per-call statistics filled by gat_count (any pointer may be NULL). */
@@ -122,7 +125,8 @@ typedef struct gat_count_stats {
     double seconds;           /* device time of the count (hipEvents)        */
     double main_kernel_ms;    /* average duration of the main engine launch  */
     uint64_t overflow_tasks;  /* tasks re-run with HBM-resident prefix sets  */
-    uint64_t alg_bytes;       /* W_alg, only when GAT_FLAG_ACCOUNT is set    */
+    uint64_t alg_bytes;       /* W_alg of the whole count (GAT_FLAG_ACCOUNT)  */
+    uint64_t alg_bytes_main;  /* W_alg of the main launch alone              */
     uint64_t tasks;           /* directed edges in the task space            */
     int32_t grid_blocks;      /* launched workgroups                         */
     int32_t block_threads;

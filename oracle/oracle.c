@@ -963,6 +963,11 @@ static void clq_rec(clq_t *Q, int d) {
 }
 
 uint64_t orc_clique_walg(int32_t n, const int64_t *off, const int32_t *adj, int k, uint64_t *walg) {
+    return orc_clique_walg_sample(n, off, adj, k, 1, 0, walg);
+}
+
+uint64_t orc_clique_walg_sample(int32_t n, const int64_t *off, const int32_t *adj, int k, int32_t stride,
+                                int32_t first, uint64_t *walg) {
     clq_t Q;
     int32_t v0, maxd = 0, v;
     int d;
@@ -974,7 +979,7 @@ uint64_t orc_clique_walg(int32_t n, const int64_t *off, const int32_t *adj, int 
     Q.off = off;
     Q.adj = adj;
     Q.k = k;
-    for (v0 = 0; v0 < n; v0++) {
+    for (v0 = first; v0 < n; v0 += stride) {
         const int32_t *n0 = adj + off[v0];
         int64_t a0 = lower_bound_i32(n0, off[v0 + 1] - off[v0], v0), i;
         for (i = 0; i < a0; i++) { /* v1 in N+(v0) */
@@ -1033,4 +1038,64 @@ void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off
     }
     free(order);
     free(nw);
+}
+
+/* Algorithmic bytes of the bitmap clique path (cliques K3..kmax, kmax <= 6)
+ * on a descending-degree graph, assuming every S fits the bit rows:
+ * per oriented edge 4(|N+(v1)| + i) + 4|S| (rows needed) + 32; per row
+ * 4(|N+(x)| + ci) + 16; 4|S|w for the K4 popcounts; 8w per (i,j) pair for
+ * K5 and per (i,j,l) triple for K6 (w = ceil(|S|/32)). Returns #K_kmax. */
+uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *adj, int kmax, uint64_t *walg) {
+    int32_t v0, maxd = 0, v;
+    uint64_t bytes = 0, cnt[7] = {0};
+    int32_t *S;
+    unsigned char *M; /* dense adjacency among S: M[i*s+j] */
+    for (v = 0; v < n; v++)
+        if (off[v + 1] - off[v] > maxd) maxd = (int32_t)(off[v + 1] - off[v]);
+    S = (int32_t *)malloc(sizeof(int32_t) * (maxd + 1));
+    M = (unsigned char *)malloc((size_t)(maxd + 1) * (maxd + 1));
+    for (v0 = 0; v0 < n; v0++) {
+        const int32_t *n0 = adj + off[v0];
+        int64_t a0 = lower_bound_i32(n0, off[v0 + 1] - off[v0], v0), i;
+        for (i = 0; i < a0; i++) {
+            int32_t v1 = n0[i];
+            const int32_t *n1 = adj + off[v1];
+            int64_t a = lower_bound_i32(n1, off[v1 + 1] - off[v1], v1), j, s = 0, w, ci, cj, cl;
+            for (j = 0; j < a; j++)
+                if (in_sorted(n0, i, n1[j])) S[s++] = n1[j];
+            bytes += 4ull * (uint64_t)(a + i) + (kmax >= 4 ? 4ull * (uint64_t)s : 0ull) + 32ull;
+            cnt[3] += (uint64_t)s;
+            if (kmax < 4 || s < 2) continue;
+            w = (s + 31) / 32;
+            for (ci = 0; ci < s; ci++) {
+                int32_t x = S[ci];
+                int64_t xl = lower_bound_i32(adj + off[x], off[x + 1] - off[x], x);
+                bytes += 4ull * (uint64_t)(xl + ci) + 16ull;
+                for (cj = 0; cj < s; cj++) M[ci * s + cj] = cj < ci && in_sorted(adj + off[x], xl, S[cj]);
+            }
+            bytes += 4ull * (uint64_t)(s * w);
+            for (ci = 0; ci < s; ci++)
+                for (cj = 0; cj < ci; cj++) {
+                    if (!M[ci * s + cj]) continue;
+                    cnt[4]++;
+                    if (kmax < 5) continue;
+                    bytes += 8ull * (uint64_t)w;
+                    for (cl = 0; cl < cj; cl++) {
+                        if (!(M[ci * s + cl] && M[cj * s + cl])) continue;
+                        cnt[5]++;
+                        if (kmax < 6) continue;
+                        bytes += 8ull * (uint64_t)w;
+                        {
+                            int64_t cm;
+                            for (cm = 0; cm < cl; cm++)
+                                if (M[ci * s + cm] && M[cj * s + cm] && M[cl * s + cm]) cnt[6]++;
+                        }
+                    }
+                }
+        }
+    }
+    free(S);
+    free(M);
+    *walg = bytes;
+    return cnt[kmax];
 }

@@ -45,6 +45,12 @@ uint64_t orc_reference_count(int32_t n, const int64_t *off, const int32_t *adj, 
 /* k-clique walk on a graph whose ids are already in descending-degree
  * order: returns #K_k and the algorithmic bytes in *walg. */
 uint64_t orc_clique_walg(int32_t n, const int64_t *off, const int32_t *adj, int k, uint64_t *walg);
+/* The same walk restricted to first vertices v0 = first, first+stride, ...
+ * (a bounded sample for CPU timing). */
+uint64_t orc_clique_walg_sample(int32_t n, const int64_t *off, const int32_t *adj, int k, int32_t stride,
+                                int32_t first, uint64_t *walg);
+/* Algorithmic bytes of the bitmap clique path for cliques K3..kmax. */
+uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *adj, int kmax, uint64_t *walg);
 /* Relabel to descending degree (ties: ascending id). Output arrays must
  * hold n+1 offsets and off[n] entries. */
 void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off_out, int32_t *adj_out);

@@ -159,6 +159,56 @@ def test_walg_matches_oracle(gat, orc, gpu):
     ro, ra = orc.relabel(off, adj)
     for k in (3, 4, 5):
         want_c, want_w = orc.clique_walg(ro, ra, k)
-        got, st = gat.Plan(g, [clique(k)], "unique").count({}, account=True)
+        got, st = gat.Plan(g, [clique(k)], "unique").count({"vertex_store": 1}, account=True)
         assert got[0] == want_c
         assert st["alg_bytes"] == want_w, (k, st["alg_bytes"], want_w)
+
+
+def test_bitmap_walg_matches_oracle(gat, orc, gpu):
+    g = gat.Graph.rmat(9, 8, 0.57, 0.19, 0.19, 3)
+    off, adj = g.csr()
+    ro, ra = orc.relabel(off, adj)
+    for kmax in (3, 4, 5, 6):
+        want_c, want_w = orc.clique_bitmap_walg(ro, ra, kmax)
+        assert want_c == orc.unique_count(off, adj, clique(kmax))
+        pats = [clique(k) for k in range(3, kmax + 1)]
+        for cap in (256, 32):  # 32: most edges go through the HBM-row kernel
+            got, st = gat.Plan(g, pats, "unique").count({"vertex_store": 2, "slot_capacity": cap}, account=True)
+            if cap == 256:
+                assert st["overflow_tasks"] == 0
+            assert got[-1] == want_c
+            assert st["alg_bytes"] == want_w, (kmax, cap, st["alg_bytes"], want_w)
+
+
+@pytest.mark.parametrize("gname", ["er200", "rmat8", "cl300"])
+def test_bitmap_clique_store(gat, orc, small_graphs, gname):
+    """Bitmap rows (vertex_store 2) against the array store and the oracle;
+    slot_capacity 32 caps |S| at 32 so larger edges take the HBM-row kernel."""
+    g = small_graphs[gname]
+    off, adj = g.csr()
+    ks = [3, 4, 5, 6]
+    want = [orc.unique_count(off, adj, clique(k)) for k in ks]
+    plan = gat.Plan(g, [clique(k) for k in ks], "unique")
+    for cfg in ({"vertex_store": 2}, {"vertex_store": 2, "slot_capacity": 32, "queue_chunk": 1},
+                {"vertex_store": 2, "gallop_ratio": 1, "search_linear_threshold": 8}, {"vertex_store": 1}):
+        say("bitmap", gname, cfg)
+        got, st = plan.count(cfg)
+        assert got == want, (cfg, got, want)
+    for sub in ([4], [5], [3, 5], [4, 6]):
+        got, _ = gat.Plan(g, [clique(k) for k in sub], "unique").count({"vertex_store": 2})
+        assert got == [want[ks.index(k)] for k in sub]
+
+
+def test_bitmap_overflow_and_labeled(gat, orc, gpu):
+    # dense graph: many S sets exceed a 32-entry cap
+    # (max |S| is 44). K4 from the oracle; K5 from the array store, which
+    # the tests above pin to the oracle (the oracle's K5 here takes ~30 s).
+    g = gat.Graph.erdos_renyi(60, 0.9, 4)
+    off, adj = g.csr()
+    arr, _ = gat.Plan(g, [clique(4), clique(5)], "unique").count({"vertex_store": 1})
+    want = [orc.unique_count(off, adj, clique(4)), arr[1]]
+    assert arr[0] == want[0]
+    got, st = gat.Plan(g, [clique(4), clique(5)], "unique").count({"vertex_store": 2, "slot_capacity": 32})
+    assert got == want and st["overflow_tasks"] > 0
+    lab, _ = gat.Plan(g, [clique(4), clique(5)], "labeled").count({"vertex_store": 2})
+    assert lab == [24 * want[0], 120 * want[1]]
