@@ -145,6 +145,14 @@ _SIGS = {
     "gat_plan_describe": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "gat_default_config": (C.c_int, [C.POINTER(KernelConfig)]),
     "gat_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+    "gat_mlp_create": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_P)]),
+    "gat_mlp_param_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "gat_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),
+    "gat_mlp_set_params": (C.c_int, [_P, _P, C.c_int64]),
+    "gat_mlp_forward": (C.c_int, [_P, _P, C.c_int32, _P]),
+    "gat_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int32, _P, C.POINTER(C.c_float)]),
+    "gat_mlp_train": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float)]),
+    "gat_mlp_free": (None, [_P]),
     "gat_copy_probe": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 }
 # Declared in include/gat.h and exported; tests check every one resolves.

@@ -1,0 +1,49 @@
+"""Driver: runs one kernel config of a workload and returns its time
+(tuning/tuning/role/driver.py:13-132 compiles a binary per trial and reads
+result files; here the engine runs in-process over the C ABI).
+
+Every run's counts must equal the first run's: a config that changes a
+count is a bug, not a fast config. An infeasible config (e.g. too much LDS)
+returns FLOAT_INF, as the reference maps a failing run (driver.py:118-122).
+"""
+import statistics
+
+from .. import core
+from .._lib import GAT_EINFEASIBLE, GatError
+
+FLOAT_INF = 1e38  # tuning/tuning/common/const.py:3
+
+
+class Driver:
+    def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1):
+        self.plan = core.Plan(graph, patterns, mode)
+        self.graph = graph
+        self.patterns = list(patterns)
+        self.reps, self.warmup = reps, warmup
+        self.counts = None
+        self.log = []
+
+    def run(self, config):
+        cfg = config.as_dict() if hasattr(config, "as_dict") else dict(config)
+        try:
+            for _ in range(self.warmup):
+                self.plan.count(cfg)
+            secs = []
+            for _ in range(self.reps):
+                c, st = self.plan.count(cfg)
+                if self.counts is None:
+                    self.counts = c
+                elif c != self.counts:
+                    raise AssertionError("config %s changed the counts: %s vs %s" % (cfg, c, self.counts))
+                secs.append(st["seconds"])
+        except GatError as e:
+            if e.code == GAT_EINFEASIBLE:
+                self.log.append((cfg, FLOAT_INF))
+                return FLOAT_INF
+            raise
+        t = statistics.median(secs)
+        self.log.append((cfg, t))
+        return t
+
+    def batch_run(self, configs):
+        return [self.run(c) for c in configs]

@@ -213,6 +213,27 @@ int gat_plan_describe(const gat_plan *p, char *buf, size_t cap);
 /* Default configuration the engine uses for zero fields. */
 int gat_default_config(gat_kernel_config *out);
 
+/* ---- learned cost model (tuner) ------------------------------------- */
+/* A 3-layer MLP in_dim -> hidden -> hidden -> 1 (ReLU) whose products run
+ * on BF16 matrix cores with FP32 accumulation; FP32 master weights, Adam.
+ * Replaces the XGBoost regressor of tuning/tuning/role/modeling.py:16-162
+ * (fit on measured runtimes, predict to rank candidate configs).
+ * Parameters are laid out W1[in][h], b1[h], W2[h][h], b2[h], W3[h], b3. */
+typedef struct gat_mlp gat_mlp;
+int gat_mlp_create(int32_t in_dim, int32_t hidden, uint64_t seed, gat_mlp **out);
+int gat_mlp_param_count(const gat_mlp *m, int64_t *out);
+int gat_mlp_get_params(gat_mlp *m, float *out, int64_t count);
+int gat_mlp_set_params(gat_mlp *m, const float *in, int64_t count);
+/* x: batch x in_dim row-major host array; y_out: batch predictions. */
+int gat_mlp_forward(gat_mlp *m, const float *x, int32_t batch, float *y_out);
+/* Gradients of the mean squared error against targets t (no update). */
+int gat_mlp_gradients(gat_mlp *m, const float *x, const float *t, int32_t batch, float *grads_out,
+                      float *loss_out);
+/* `steps` full-batch Adam steps; loss_out gets the last step's loss. */
+int gat_mlp_train(gat_mlp *m, const float *x, const float *t, int32_t batch, int32_t steps, float lr,
+                  float *loss_out);
+void gat_mlp_free(gat_mlp *m);
+
 /* ---- device probes --------------------------------------------------- */
 /* Number of visible HIP devices (0 when none); never initialises more than
  * hipGetDeviceCount does. */
