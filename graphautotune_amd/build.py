@@ -58,6 +58,12 @@ def build_lib(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)
+    # pm command-line tool next to the library (links libgat.so by rpath)
+    os.makedirs(os.path.join(PKG, "bin"), exist_ok=True)
+    pm = os.path.join(PKG, "bin", "pm")
+    _run([CXX, "-O2", "-std=c++20", "-I" + INCLUDE, os.path.join(CSRC, "pm_main.cpp"), "-o", pm + ".tmp",
+          "-L" + PKG, "-lgat", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(pm + ".tmp", pm)
     return LIB
 
 

@@ -61,6 +61,7 @@ struct WaveState {
     unsigned long long wave_acc[GATP_MAX_TERMS];
     long long sbeg[GATP_MAX_SLOTS];
     int slen[GATP_MAX_SLOTS];
+    int staged[GATP_MAX_SLOTS];  // neighbour list copied into its LDS buffer
     int val[GATP_MAX_DEPTH + 1];
     int f_node[GATP_MAX_DEPTH + 1];
     int f_cur[GATP_MAX_DEPTH + 1];
@@ -229,8 +230,15 @@ __device__ __forceinline__ bool stream_a(const KArgs &a, int alen, int blen, boo
 template <bool GS, class F>
 __device__ __forceinline__ auto with_slot(const Ctx &C, int slot, F f) {
     const GatSlot &s = C.P.slots[slot];
-    if (s.kind == GATP_ALIAS) return f(C.a.adj + C.S->sbeg[slot]);
+    if (s.kind == GATP_ALIAS && (GS || !C.S->staged[slot])) return f(C.a.adj + C.S->sbeg[slot]);
     return f(C.store + (long long)s.store * C.a.cap);
+}
+// Is the slot's content in this wave's LDS buffers?
+template <bool GS>
+__device__ __forceinline__ bool in_lds(const Ctx &C, int slot) {
+    if (GS) return false;
+    const GatSlot &s = C.P.slots[slot];
+    return s.kind == GATP_LIST || C.S->staged[slot];
 }
 
 // Evaluate one op for the wave-uniform candidate c at depth d.
@@ -244,9 +252,16 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
     int alen = dup ? 0 : uni(trunc_len(a, c, ab, deg, t));
     const int slot = o.slot;
     if (o.kind == GATP_ALIAS) {
+        // a list that later ops search is staged into LDS when it fits
+        const bool stage = !GS && o.store >= 0 && alen <= a.cap;
+        if (stage) {
+            int *dst = C.store + (long long)o.store * a.cap;
+            for (int k = C.ln; k < alen; k += W) dst[k] = a.adj[ab + k];
+        }
         if (C.ln == 0) {
             S->sbeg[slot] = ab;
             S->slen[slot] = alen;
+            S->staged[slot] = stage ? 1 : 0;
         }
         return;
     }
@@ -260,16 +275,9 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
     const bool has_r = f >= 0;
     bool b_lds = false;
     if (has_r) {
-        const GatSlot &fs = C.P.slots[f];
-        b_lds = fs.kind == GATP_LIST && !GS;
+        b_lds = in_lds<GS>(C, f);
         const int flen = S->slen[f];
-        if (fs.kind == GATP_ALIAS) {
-            const int *B = a.adj + S->sbeg[f];
-            blen = dup ? 0 : (t == INT_MAX ? flen : uni(lbound(B, flen, t)));
-        } else {
-            const int *B = C.store + (long long)fs.store * a.cap;
-            blen = dup ? 0 : (t == INT_MAX ? flen : uni(lbound(B, flen, t)));
-        }
+        blen = dup ? 0 : (t == INT_MAX ? flen : uni(with_slot<GS>(C, f, [&](auto B) { return lbound(B, flen, t); })));
     }
     if (o.kind == GATP_LIST) {
         const int ub = has_r ? min(alen, blen) : alen;
@@ -285,9 +293,10 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
         result = o.kind == GATP_LIST ? wave_isect<true>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln)
                                      : wave_isect<false>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln);
     } else {
-        const GatSlot &fs = C.P.slots[f];
-        if (fs.kind == GATP_ALIAS) {
-            const int *B = a.adj + S->sbeg[f];
+        // the father lives in HBM: a neighbour list, or (overflow launch) a set buffer
+        if (!b_lds) {
+            const int *B = C.P.slots[f].kind == GATP_ALIAS ? a.adj + S->sbeg[f]
+                                                            : C.store + (long long)C.P.slots[f].store * a.cap;
             if (o.kind == GATP_LIST)
                 result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
                             : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
@@ -295,7 +304,7 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
                 result = sa ? wave_isect<false>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
                             : wave_isect<false>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
         } else {
-            const int *B = C.store + (long long)fs.store * a.cap;
+            const int *B = C.store + (long long)C.P.slots[f].store * a.cap;
             if (o.kind == GATP_LIST)
                 result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
                             : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
@@ -415,10 +424,10 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
             const int *Bl = C.store;
             if (has_r) {
                 const GatSlot &fs = C.P.slots[f];
-                b_lds = fs.kind == GATP_LIST && !GS;
-                if (fs.kind == GATP_ALIAS) Bg = a.adj + S->sbeg[f];
-                else if (GS) Bg = C.store + (long long)fs.store * a.cap;
-                else Bl = C.store + (long long)fs.store * a.cap;
+                b_lds = in_lds<GS>(C, f);
+                if (b_lds) Bl = C.store + (long long)fs.store * a.cap;
+                else if (fs.kind == GATP_ALIAS) Bg = a.adj + S->sbeg[f];
+                else if constexpr (GS) Bg = C.store + (long long)fs.store * a.cap;
                 const int flen = S->slen[f];
                 if (pass && !dup) blen = t == INT_MAX ? flen : (b_lds ? lbound(Bl, flen, t) : lbound(Bg, flen, t));
             }

@@ -442,10 +442,18 @@ int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *ou
     int maxd = 0;
     for (auto &h : C.nodes) maxd = std::max(maxd, h.depth);
     std::vector<int> per_depth(maxd + 1, 0);
+    // A neighbour list that later ops search (it is some op's father) gets an
+    // LDS buffer too: it is staged there when it fits.
+    std::vector<char> searched(C.slots.size(), 0);
+    for (size_t i = 0; i < C.slots.size(); i++)
+        if (!dead[i] && C.slots[i].father >= 0) searched[C.slots[i].father] = 1;
+    auto needs_store = [&](int sid) {
+        return C.slots[sid].kind == GATP_LIST || (C.slots[sid].kind == GATP_ALIAS && searched[sid]);
+    };
     for (auto &h : C.nodes) {
         int k = 0;
         for (int sid : h.ops)
-            if (C.slots[sid].kind == GATP_LIST) k++;
+            if (needs_store(sid)) k++;
         per_depth[h.depth] = std::max(per_depth[h.depth], k);
     }
     std::vector<int> store_base(maxd + 2, 0);
@@ -493,7 +501,7 @@ int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *ou
             gs.node = int16_t(ii);
             gs.leaf_idx = -1;
             gs.store = -1;
-            if (s.kind == GATP_LIST) o.store = gs.store = int8_t(store_base[h.depth] + lists++);
+            if (needs_store(sid)) o.store = gs.store = int8_t(store_base[h.depth] + lists++);
             if ((s.kind == GATP_COUNT || s.kind == GATP_ALIAS) && g.leaf) o.leaf_idx = gs.leaf_idx = int8_t(leafc++);
         }
         if (leafc > GATP_MAX_LEAF_OPS) GAT_FAIL(GAT_ELIMIT, "leaf needs %d counters (> %d)", leafc, GATP_MAX_LEAF_OPS);
