@@ -1,0 +1,111 @@
+"""Tuner host logic (CPU only): config space, fingerprints, simulated
+annealing, grid search and the driver's infeasible-config mapping, with a
+stub engine and a stub cost model."""
+import hashlib
+import random
+
+import numpy as np
+import pytest
+
+from graphautotune_amd.tuning import space as sp
+from graphautotune_amd.tuning.driver import FLOAT_INF
+from graphautotune_amd.tuning.manipulator import Manipulator
+from graphautotune_amd.tuning.tuner import Tuner, grid_search
+
+
+def test_space_matches_reference_values():
+    # tuning/tuning/config/details.py:11-13 (SetSearchType) kept verbatim
+    assert sp.BY_NAME["search_linear_threshold"].values == [0, 4, 16, 64, 2 ** 31 - 1]
+    full = sp.ConfigSpace()
+    assert full.size() == 3 * 4 * 4 * 4 * 3 * 3 * 5 * 2
+    assert len(list(sp.ConfigSpace(["block_waves", "queue_chunk"]).all())) == 12
+
+
+def test_fingerprint_is_md5_of_sorted_index_vector():
+    # base.py:218-222,259-262: indices of the values, keys sorted by name
+    c = sp.Config({"queue_chunk": 64, "block_waves": 2})
+    idx = [sp.BY_NAME[k].values.index(c.values[k]) for k in sorted(c.values)]
+    assert c.get_list() == idx
+    assert c.fingerprint() == hashlib.md5(",".join(map(str, idx)).encode()).hexdigest()
+    with pytest.raises(ValueError):
+        sp.Config({"queue_chunk": 3})
+
+
+def test_random_walk_changes_exactly_one_knob():
+    s = sp.ConfigSpace()
+    rng = random.Random(1)
+    c = s.random(rng)
+    for _ in range(50):
+        d = s.random_walk(c, rng)
+        assert sum(c.values[k] != d.values[k] for k in c.values) == 1
+        c = d
+
+
+class StubDriver:
+    """Runtime surface with a known optimum; some configs infeasible."""
+
+    def __init__(self):
+        self.calls = 0
+
+    def cost(self, c):
+        v = c.values
+        if v["block_waves"] == 4 and v["slot_capacity"] == 1024:
+            return FLOAT_INF  # e.g. too much LDS
+        return (1.0 + abs(v["waves_per_cu"] - 12) / 4 + abs(np.log2(v["queue_chunk"]) - 2) / 2
+                + (v["slot_capacity"] != 256) * 0.5 + (v["block_waves"] != 1) * 0.25)
+
+    def run(self, c):
+        self.calls += 1
+        return self.cost(c)
+
+    def batch_run(self, cs):
+        return [self.run(c) for c in cs]
+
+
+class StubModel:
+    """Perfect-knowledge cost model: checks the search logic, not learning."""
+
+    def __init__(self, drv):
+        self.drv, self.xs, self.ys = drv, [], []
+
+    def predict_list(self, cs):
+        return np.array([min(self.drv.cost(c), 1e9) for c in cs])
+
+    def is_config_exist(self, c):
+        return c.fingerprint() in {x.fingerprint() for x in self.xs}
+
+    def update_list(self, xs, ys):
+        self.xs += xs
+        self.ys += ys
+
+    def get_best_config(self):
+        i = int(np.argmin(self.ys))
+        return self.xs[i], self.ys[i]
+
+
+def test_sa_tuner_finds_optimum_with_a_good_model():
+    space = sp.ConfigSpace(["block_waves", "waves_per_cu", "queue_chunk", "slot_capacity"])
+    drv = StubDriver()
+    m = Manipulator(StubModel(drv), drv, space, seed=3)
+    best, t = Tuner(m, max_round=6, batch_size=5, warmup_examples=8).tune()
+    assert t == pytest.approx(1.0)
+    assert best.values["waves_per_cu"] == 12 and best.values["queue_chunk"] == 4
+    assert drv.calls <= 8 + 6 * 5  # every pick is measured, nothing more
+
+
+def test_grid_search_is_the_judge():
+    space = sp.ConfigSpace(["block_waves", "waves_per_cu", "queue_chunk", "slot_capacity"])
+    drv = StubDriver()
+    best, t, results = grid_search(drv, space)
+    assert len(results) == space.size()
+    assert t == pytest.approx(1.0)
+    assert sum(1 for _, y in results if y == FLOAT_INF) == 4 * 4  # infeasible corner skipped, not fatal
+
+
+def test_manipulator_drops_infeasible_runs():
+    space = sp.ConfigSpace(["block_waves", "slot_capacity"])
+    drv = StubDriver()
+    m = Manipulator(StubModel(drv), drv, space)
+    bad = sp.Config({"block_waves": 4, "slot_capacity": 1024})
+    xs, ys = m.run_batch([bad, sp.Config()])
+    assert xs == [sp.Config()] and ys[0] < FLOAT_INF
