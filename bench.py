@@ -99,6 +99,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", type=str, default=None, help="JSON kernel config for the batched run")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="time only the batched plan (profiling runs: one kernel config per kernel name)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the device copy kernel over MIB MiB (3 timed launches) and exit; "
+                         "calibrates the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -106,6 +111,13 @@ def main():
         sys.exit(2)
 
     import graphautotune_amd as gat
+    if args.copy_probe:
+        import ctypes as C
+        from graphautotune_amd._lib import check, lib
+        gbps = C.c_double()
+        check(lib().gat_copy_probe(args.copy_probe << 20, 3, C.byref(gbps)))
+        print(json.dumps({"copy_probe_mib": args.copy_probe, "launches": 4, "gbps": gbps.value}), flush=True)
+        return
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -124,33 +136,41 @@ def main():
     plain4 = gat.Plan(g, [K4], "unique")
     plain5 = gat.Plan(g, [K5], "unique")
 
+    plains = [] if args.no_plain else [plain4, plain5]
     for _ in range(args.warmup):
         batched.count(cfg)
-        plain4.count({})
-        plain5.count({})
+        for p in plains:
+            p.count({})
     # instrumented pass: algorithmic bytes (not timed)
     acc_counts, acc = batched.count(cfg, account=True)
     walg = acc["alg_bytes"]
     walg_main = acc["alg_bytes_main"]
-    _, acc4 = plain4.count({}, account=True)
-    _, acc5 = plain5.count({}, account=True)
-    log("W_alg batched %.4g B, plain %.4g + %.4g B" % (walg, acc4["alg_bytes"], acc5["alg_bytes"]))
+    plain_walg = None
+    if plains:
+        plain_walg = sum(p.count({}, account=True)[1]["alg_bytes"] for p in plains)
+        log("W_alg batched %.4g B, plain %.4g B" % (walg, plain_walg))
 
     sync()
     tw0 = time.perf_counter()
     counts, secs, mains, walls, st = timed(batched, cfg, args.steps)
     sync()
     wall = time.perf_counter() - tw0
-    c4, s4, _, _, _ = timed(plain4, {}, args.steps)
-    c5, s5, _, _, _ = timed(plain5, {}, args.steps)
-    if counts != acc_counts or counts != [c4[0], c5[0]]:
-        raise AssertionError("batched %s, instrumented %s and plain %s counts differ" % (counts, acc_counts, c4 + c5))
+    if counts != acc_counts:
+        raise AssertionError("batched %s and instrumented %s counts differ" % (counts, acc_counts))
     t_b = statistics.median(secs)
-    t_plain = statistics.median(s4) + statistics.median(s5)
     main_ms = statistics.median(mains)
     achieved = walg_main / (main_ms * 1e-3) / 1e9
-    log("counts K4 %d K5 %d | batched %.3f ms (main %.3f ms), plain %.3f ms | overflow tasks %d"
-        % (counts[0], counts[1], t_b * 1e3, main_ms, t_plain * 1e3, st["overflow_tasks"]))
+    t_plain, plain_ms = None, None
+    if plains:
+        c4, s4, _, _, _ = timed(plain4, {}, args.steps)
+        c5, s5, _, _, _ = timed(plain5, {}, args.steps)
+        if counts != [c4[0], c5[0]]:
+            raise AssertionError("batched %s and plain %s counts differ" % (counts, c4 + c5))
+        plain_ms = {"K4": statistics.median(s4) * 1e3, "K5": statistics.median(s5) * 1e3}
+        t_plain = statistics.median(s4) + statistics.median(s5)
+    log("counts K4 %d K5 %d | batched %.3f ms (main %.3f ms), plain %s ms | overflow tasks %d"
+        % (counts[0], counts[1], t_b * 1e3, main_ms, "%.3f" % (t_plain * 1e3) if t_plain else "-",
+           st["overflow_tasks"]))
 
     kernel_name = "gatk::clique_kernel" if cfg.get("vertex_store", 0) != 1 else "gatk::engine_kernel<false>"
     traffic = None
@@ -184,9 +204,9 @@ def main():
         "counts": {"K4": counts[0], "K5": counts[1]},
         "reference_mode_counts": {"K4": 4 * counts[0], "K5": 5 * counts[1]},
         "device_ms_batched": t_b * 1e3,
-        "device_ms_plain": {"K4": statistics.median(s4) * 1e3, "K5": statistics.median(s5) * 1e3},
-        "plain_value": E / t_plain,
-        "batched_vs_plain": t_plain / t_b,
+        "device_ms_plain": plain_ms,
+        "plain_value": E / t_plain if t_plain else None,
+        "batched_vs_plain": t_plain / t_b if t_plain else None,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -202,7 +222,9 @@ def main():
         },
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, acc4["alg_bytes"] + acc5["alg_bytes"], args.cpu_budget)
+        if plain_walg is None:
+            plain_walg = sum(gat.Plan(g, [p], "unique").count({}, account=True)[1]["alg_bytes"] for p in (K4, K5))
+        out["cpu_baseline"] = cpu_baseline(g, plain_walg, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 

@@ -77,11 +77,13 @@ class KernelConfig(C.Structure):
         ("search_linear_threshold", C.c_int32),
         ("max_regs", C.c_int32),
         ("vertex_store", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("hub_degree", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     KNOBS = ("block_waves", "waves_per_cu", "queue_chunk", "slot_capacity", "leaf_mode",
-             "intersect_strategy", "gallop_ratio", "search_linear_threshold", "max_regs", "vertex_store")
+             "intersect_strategy", "gallop_ratio", "search_linear_threshold", "max_regs", "vertex_store",
+             "hub_degree")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.KNOBS}
@@ -109,6 +111,17 @@ class CountStats(C.Structure):
     ]
 
 
+class PlanOptions(C.Structure):
+    _fields_ = [
+        ("schedule_policy", C.c_int32),
+        ("schedule_rank", C.c_int32),
+        ("reserved", C.c_int64 * 3),
+    ]
+
+
+SCHEDULE_REFERENCE, SCHEDULE_GRAPH = 0, 1
+
+
 class GatError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("gat error %d: %s" % (code, msg))
@@ -120,6 +133,8 @@ _SIGS = {
     "gat_last_error": (C.c_char_p, []),
     "gat_version": (C.c_char_p, []),
     "gat_schedule_build": (C.c_int, [C.c_char_p, C.POINTER(ScheduleData), C.c_size_t]),
+    "gat_schedule_candidates": (C.c_int, [C.c_char_p, C.c_int32, C.c_int64, C.c_int64, _P, C.c_size_t,
+                                          C.c_int32, C.POINTER(C.c_int32)]),
     "gat_pattern_automorphisms": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
     "gat_graph_load_bin": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "gat_graph_load_txt": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
@@ -134,8 +149,11 @@ _SIGS = {
     "gat_graph_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "gat_graph_csr": (C.c_int, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "gat_graph_degree_histogram": (C.c_int, [_P, _P]),
+    "gat_graph_triangles": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "gat_graph_free": (None, [_P]),
     "gat_plan_create": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "gat_plan_create_ex": (C.c_int, [_P, C.POINTER(C.c_char_p), C.c_int32, C.c_int32, C.POINTER(PlanOptions),
+                                     C.POINTER(_P)]),
     "gat_plan_schedule": (C.c_int, [_P, C.c_int32, C.POINTER(ScheduleData), C.c_size_t]),
     "gat_count": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_uint32, C.POINTER(C.c_uint64),
                             C.POINTER(CountStats)]),

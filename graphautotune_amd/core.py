@@ -17,8 +17,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (MODE_LABELED, MODE_REFERENCE, MODE_UNIQUE, CountStats, KernelConfig,
-                   ScheduleData, check, lib)
+from ._lib import (MODE_LABELED, MODE_REFERENCE, MODE_UNIQUE, SCHEDULE_GRAPH, SCHEDULE_REFERENCE,
+                   CountStats, KernelConfig, PlanOptions, ScheduleData, check, lib)
 
 CPU_DEVICE, GPU_DEVICE, UNKNOWN_DEVICE = 0, 1, 2  # consts/general_consts.hpp:6-10
 
@@ -54,6 +54,18 @@ class Pattern:
 
     def __repr__(self):
         return "Pattern(%r)" % self.adj
+
+
+def schedule_candidates(pattern, v_cnt=5000, e_cnt=1000000, tri_cnt=10000, limit=64):
+    """Schedule candidates ranked by the cost estimate under the given graph
+    statistics (the defaults are the reference's fixed numbers, under which
+    entry 0 is Schedule(pattern)). Returns (list of Schedule, total)."""
+    adj = Pattern(pattern).adj
+    buf = (ScheduleData * max(1, limit))()
+    n = C.c_int32()
+    check(lib().gat_schedule_candidates(adj.encode(), int(v_cnt), int(e_cnt), int(tri_cnt), buf,
+                                        C.sizeof(ScheduleData), int(limit), C.byref(n)))
+    return [Schedule.from_data(adj, buf[i]) for i in range(min(limit, n.value))], n.value
 
 
 class Schedule:
@@ -193,6 +205,12 @@ class GlobalMemoryGraph:
                else np.zeros(0, dtype=np.int32))
         return off, adj
 
+    def triangles(self):
+        """Number of triangles (host count, cached by the library)."""
+        out = C.c_int64()
+        check(lib().gat_graph_triangles(self._h, C.byref(out)))
+        return out.value
+
     def degree_histogram(self):
         bins = np.zeros(32, dtype=np.int64)
         check(lib().gat_graph_degree_histogram(self._h, bins.ctypes.data))
@@ -205,18 +223,30 @@ class GlobalMemoryGraph:
 Graph = GlobalMemoryGraph
 
 
-class Plan:
-    """One or more patterns compiled for one graph (gat_plan)."""
+SCHEDULES = {"reference": SCHEDULE_REFERENCE, "graph": SCHEDULE_GRAPH}
 
-    def __init__(self, graph, patterns, mode="unique"):
+
+class Plan:
+    """One or more patterns compiled for one graph (gat_plan).
+
+    schedule="reference" uses the reference's Schedule for every pattern;
+    schedule="graph" (unique/labeled modes) ranks the Schedule candidates by
+    the reference's cost estimate under this graph's statistics and takes
+    the `schedule_rank`-th."""
+
+    def __init__(self, graph, patterns, mode="unique", schedule="reference", schedule_rank=0):
         if isinstance(patterns, (str, Pattern)):
             patterns = [patterns]
         self.patterns = [Pattern(p).adj for p in patterns]
         self.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         self.graph = graph  # keep alive
         arr = (C.c_char_p * len(self.patterns))(*[p.encode() for p in self.patterns])
+        opt = PlanOptions()
+        opt.schedule_policy = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
+        opt.schedule_rank = int(schedule_rank)
         h = C.c_void_p()
-        check(lib().gat_plan_create(graph.handle, arr, len(self.patterns), self.mode, C.byref(h)))
+        check(lib().gat_plan_create_ex(graph.handle, arr, len(self.patterns), self.mode, C.byref(opt),
+                                       C.byref(h)))
         self._h = h
 
     def __del__(self):

@@ -54,6 +54,30 @@ int gat_schedule_build(const char *adj_str, gat_schedule_data *out, size_t cap) 
     GUARD_END
 }
 
+int gat_schedule_candidates(const char *adj_str, int32_t v_cnt, int64_t e_cnt, int64_t tri_cnt,
+                            gat_schedule_data *out, size_t elem_size, int32_t cap, int32_t *n_out) {
+    if (!adj_str || !n_out || cap < 0 || (cap > 0 && !out)) GAT_FAIL(GAT_EINVAL, "bad arguments");
+    if (cap > 0 && elem_size < sizeof(gat_schedule_data))
+        GAT_FAIL(GAT_EINVAL, "schedule record too small (%zu < %zu)", elem_size, sizeof(gat_schedule_data));
+    if (v_cnt <= 0 || e_cnt <= 0 || tri_cnt < 0) GAT_FAIL(GAT_EINVAL, "graph statistics must be positive");
+    GUARD_BEGIN
+    Pattern p;
+    int rc = parse_pattern(adj_str, &p);
+    if (rc) return rc;
+    std::vector<ScheduleCandidate> c;
+    rc = schedule_candidates(p, ScheduleStats{v_cnt, e_cnt, tri_cnt}, &c);
+    if (rc) return rc;
+    *n_out = (int32_t)c.size();
+    for (int32_t i = 0; i < cap && i < (int32_t)c.size(); i++) {
+        Schedule s;
+        rc = build_schedule_from(p, c[i], &s);
+        if (rc) return rc;
+        export_schedule(s, reinterpret_cast<gat_schedule_data *>(reinterpret_cast<char *>(out) + i * elem_size));
+    }
+    return GAT_OK;
+    GUARD_END
+}
+
 int gat_pattern_automorphisms(const char *adj_str, int64_t *out) {
     if (!adj_str || !out) GAT_FAIL(GAT_EINVAL, "null argument");
     GUARD_BEGIN
@@ -158,7 +182,31 @@ int gat_graph_degree_histogram(const gat_graph *g, int64_t *bins) {
 
 void gat_graph_free(gat_graph *g) { delete g; }
 
-int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_patterns, int32_t mode, gat_plan **out) {
+static std::shared_ptr<Graph> relabeled_graph(gat_graph *g) {  // caller holds g->mu
+    if (!g->relabeled) {
+        auto r = std::make_shared<Graph>();
+        relabel_by_degree(*g->g, r.get(), nullptr);
+        g->relabeled = r;
+    }
+    return g->relabeled;
+}
+
+static int64_t graph_triangles(gat_graph *g) {  // caller holds g->mu
+    if (g->triangles < 0) g->triangles = count_triangles(*relabeled_graph(g));
+    return g->triangles;
+}
+
+int gat_graph_triangles(gat_graph *g, int64_t *out) {
+    if (!g || !out) GAT_FAIL(GAT_EINVAL, "null argument");
+    GUARD_BEGIN
+    std::lock_guard<std::mutex> lk(g->mu);
+    *out = graph_triangles(g);
+    return GAT_OK;
+    GUARD_END
+}
+
+int gat_plan_create_ex(gat_graph *g, const char *const *adj_strs, int32_t n_patterns, int32_t mode,
+                       const gat_plan_options *opt, gat_plan **out) {
     if (!g || !adj_strs || n_patterns < 1 || !out) GAT_FAIL(GAT_EINVAL, "bad plan arguments");
     GUARD_BEGIN
     std::vector<std::string> adj;
@@ -166,26 +214,37 @@ int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_pattern
         if (!adj_strs[i]) GAT_FAIL(GAT_EINVAL, "null pattern %d", i);
         adj.emplace_back(adj_strs[i]);
     }
+    PlanOptions po;
+    if (opt) {
+        po.schedule_policy = opt->schedule_policy;
+        po.schedule_rank = opt->schedule_rank;
+        if (po.schedule_policy == GAT_SCHEDULE_GRAPH && mode != GAT_MODE_REFERENCE) {
+            std::lock_guard<std::mutex> lk(g->mu);
+            const int64_t tri = graph_triangles(g);
+            if (g->g->n <= 0 || g->g->m <= 0)
+                GAT_FAIL(GAT_EINVAL, "graph-aware schedules need a graph with edges");
+            po.stats = ScheduleStats{g->g->n, g->g->m, std::max<int64_t>(tri, 1)};
+        }
+    }
     auto *p = new gat_plan();
-    int rc = compile_plan(adj, mode, &p->cp);
+    int rc = compile_plan(adj, mode, po, &p->cp);
     if (rc) {
         delete p;
         return rc;
     }
     if (p->cp.relabel) {
         std::lock_guard<std::mutex> lk(g->mu);
-        if (!g->relabeled) {
-            auto r = std::make_shared<Graph>();
-            relabel_by_degree(*g->g, r.get(), nullptr);
-            g->relabeled = r;
-        }
-        p->graph = g->relabeled;
+        p->graph = relabeled_graph(g);
     } else {
         p->graph = g->g;
     }
     *out = p;
     return GAT_OK;
     GUARD_END
+}
+
+int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_patterns, int32_t mode, gat_plan **out) {
+    return gat_plan_create_ex(g, adj_strs, n_patterns, mode, nullptr, out);
 }
 
 int gat_plan_schedule(const gat_plan *p, int32_t i, gat_schedule_data *out, size_t cap) {
@@ -209,6 +268,7 @@ int gat_default_config(gat_kernel_config *out) {
     out->gallop_ratio = 32;
     out->search_linear_threshold = 0;
     out->max_regs = 0;
+    out->hub_degree = 256;
     return GAT_OK;
 }
 

@@ -24,6 +24,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "handles.hpp"
@@ -54,6 +55,14 @@ struct KArgs {
     int gallop_ratio;
     int strategy;
     int account;
+    // Dense neighbourhood bitmaps of high-degree vertices ("hubs"): row h
+    // holds bit x set iff x is a neighbour of hub h. hub_map == nullptr means
+    // the graph is degree-ordered and the hubs are vertices [0, n_hubs).
+    const unsigned *hub_bits;
+    const int *hub_map;
+    long long hub_words;  // words per row: (n + 31) / 32
+    int n_hubs;
+    int pad_;
 };
 
 struct WaveState {
@@ -194,12 +203,27 @@ __device__ __forceinline__ int trunc_len(const KArgs &a, int v, long long beg, i
     return lbound(a.adj + beg, deg, t);
 }
 
-// Wave-cooperative intersection of stream[0,slen) with search[0,rlen),
-// skipping the (at most 8) excluded values; writes the sorted result to out
-// when MAT. has_r=false means "no second operand": every element passes.
-template <bool MAT, class SP, class RP, class OP>
-__device__ __forceinline__ int wave_isect(SP stream, int slen, RP search, int rlen, bool has_r, OP out,
-                                          const int *ex, int nex, int lin_thr, int ln) {
+// Hub row of vertex v, or -1.
+__device__ __forceinline__ int hub_of(const KArgs &a, int v) {
+    if (a.n_hubs == 0) return -1;
+    if (a.hub_map) return a.hub_map[v];
+    return v < a.n_hubs ? v : -1;
+}
+// Membership of x in the first len entries of hub h's sorted neighbour
+// list, whose last kept entry is lim: one bitmap word instead of a search.
+__device__ __forceinline__ bool hub_has(const KArgs &a, int h, int x, int lim) {
+    return x <= lim && ((a.hub_bits[(long long)h * a.hub_words + (x >> 5)] >> (x & 31)) & 1u);
+}
+// Largest kept entry of a neighbour-list prefix of length len (-1 if empty).
+__device__ __forceinline__ int list_lim(const KArgs &a, long long beg, int len) {
+    return len > 0 ? a.adj[beg + len - 1] : -1;
+}
+
+// Wave-cooperative intersection of stream[0,slen) with the set tested by
+// probe(x), skipping the (at most 8) excluded values; writes the sorted
+// result to out when MAT.
+template <bool MAT, class SP, class PR, class OP>
+__device__ __forceinline__ int wave_isect(SP stream, int slen, PR probe, OP out, const int *ex, int nex, int ln) {
     int total = 0;
     for (int base = 0; base < slen; base += W) {
         const int i = base + ln;
@@ -207,7 +231,7 @@ __device__ __forceinline__ int wave_isect(SP stream, int slen, RP search, int rl
         int x = 0;
         if (i < slen) {
             x = stream[i];
-            hit = !has_r || has(search, rlen, x, lin_thr);
+            hit = probe(x);
             for (int k = 0; k < nex; k++) hit &= (x != ex[k]);
         }
         const unsigned long long m = ballot(hit);
@@ -217,11 +241,16 @@ __device__ __forceinline__ int wave_isect(SP stream, int slen, RP search, int rl
     return total;
 }
 
-// Stream A (the new vertex's neighbour list) or the father set?
-__device__ __forceinline__ bool stream_a(const KArgs &a, int alen, int blen, bool b_in_lds) {
+// Stream A (the new vertex's neighbour list) and probe the father set, or
+// the reverse? A side is cheap to probe when it sits in LDS or has a hub
+// bitmap; the other side is then streamed unless it is gallop_ratio times
+// longer.
+__device__ __forceinline__ bool stream_a(const KArgs &a, int alen, int blen, bool b_fast, bool a_fast) {
     if (a.strategy == 1) return true;
     if (a.strategy == 2) return alen <= blen;
-    if (b_in_lds) return !((long long)alen > (long long)a.gallop_ratio * (long long)blen);
+    const long long r = a.gallop_ratio;
+    if (b_fast && !a_fast) return !((long long)alen > r * (long long)blen);
+    if (a_fast && !b_fast) return (long long)blen > r * (long long)alen;
     return alen <= blen;
 }
 
@@ -274,10 +303,17 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
     int blen = 0, result = 0;
     const bool has_r = f >= 0;
     bool b_lds = false;
+    int hub_b = -1;
+    long long bbeg = 0;
     if (has_r) {
+        const GatSlot &fs = C.P.slots[f];
         b_lds = in_lds<GS>(C, f);
         const int flen = S->slen[f];
         blen = dup ? 0 : (t == INT_MAX ? flen : uni(with_slot<GS>(C, f, [&](auto B) { return lbound(B, flen, t); })));
+        if (fs.kind == GATP_ALIAS) {
+            bbeg = S->sbeg[f];
+            if (!b_lds) hub_b = uni(hub_of(a, S->val[fs.depth]));
+        }
     }
     if (o.kind == GATP_LIST) {
         const int ub = has_r ? min(alen, blen) : alen;
@@ -288,29 +324,40 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
     }
     const int *A = a.adj + ab;
     int *out = C.store + (long long)(o.store < 0 ? 0 : o.store) * a.cap;
-    const bool sa = !has_r || stream_a(a, alen, blen, b_lds);
+    const int hub_a = has_r ? uni(hub_of(a, c)) : -1;
+    const bool sa = !has_r || stream_a(a, alen, blen, b_lds || hub_b >= 0, hub_a >= 0);
+    const int lin = a.lin_thr;
+    auto run = [&](auto stream, int slen, auto probe) {
+        return o.kind == GATP_LIST ? wave_isect<true>(stream, slen, probe, out, ex, nex, C.ln)
+                                   : wave_isect<false>(stream, slen, probe, out, ex, nex, C.ln);
+    };
     if (!has_r) {
-        result = o.kind == GATP_LIST ? wave_isect<true>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln)
-                                     : wave_isect<false>(A, alen, A, 0, false, out, ex, nex, a.lin_thr, C.ln);
-    } else {
-        // the father lives in HBM: a neighbour list, or (overflow launch) a set buffer
-        if (!b_lds) {
-            const int *B = C.P.slots[f].kind == GATP_ALIAS ? a.adj + S->sbeg[f]
-                                                            : C.store + (long long)C.P.slots[f].store * a.cap;
-            if (o.kind == GATP_LIST)
-                result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
-                            : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
-            else
-                result = sa ? wave_isect<false>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
-                            : wave_isect<false>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
-        } else {
+        result = run(A, alen, [](int) { return true; });
+    } else if (sa) {
+        // stream A, probe the father: LDS buffer, hub bitmap, or HBM list
+        if (b_lds) {
             const int *B = C.store + (long long)C.P.slots[f].store * a.cap;
-            if (o.kind == GATP_LIST)
-                result = sa ? wave_isect<true>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
-                            : wave_isect<true>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
-            else
-                result = sa ? wave_isect<false>(A, alen, B, blen, true, out, ex, nex, a.lin_thr, C.ln)
-                            : wave_isect<false>(B, blen, A, alen, true, out, ex, nex, a.lin_thr, C.ln);
+            result = run(A, alen, [&](int x) { return has(B, blen, x, lin); });
+        } else if (hub_b >= 0) {
+            const int lim = uni(list_lim(a, bbeg, blen));
+            result = run(A, alen, [&](int x) { return hub_has(a, hub_b, x, lim); });
+        } else {
+            const int *B = C.P.slots[f].kind == GATP_ALIAS ? a.adj + bbeg
+                                                           : C.store + (long long)C.P.slots[f].store * a.cap;
+            result = run(A, alen, [&](int x) { return has(B, blen, x, lin); });
+        }
+    } else {
+        // stream the father, probe A
+        auto probe_a = [&](int x) { return has(A, alen, x, lin); };
+        const int lim_a = hub_a >= 0 ? uni(list_lim(a, ab, alen)) : 0;
+        auto probe_hub = [&](int x) { return hub_has(a, hub_a, x, lim_a); };
+        if (b_lds) {
+            const int *B = C.store + (long long)C.P.slots[f].store * a.cap;
+            result = hub_a >= 0 ? run(B, blen, probe_hub) : run(B, blen, probe_a);
+        } else {
+            const int *B = C.P.slots[f].kind == GATP_ALIAS ? a.adj + bbeg
+                                                           : C.store + (long long)C.P.slots[f].store * a.cap;
+            result = hub_a >= 0 ? run(B, blen, probe_hub) : run(B, blen, probe_a);
         }
     }
     if (C.ln == 0) {
@@ -419,9 +466,11 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
             bool b_lds = false;
             int blen = 0;
             // Bg: father in HBM (a neighbour list, or any set in the overflow
-            // launch); Bl: father in this wave's LDS buffers.
+            // launch); Bl: father in this wave's LDS buffers. hub_b: the
+            // father is a hub's neighbour list in HBM, probed by bitmap.
             const int *Bg = a.adj;
             const int *Bl = C.store;
+            int hub_b = -1, lim_b = 0;
             if (has_r) {
                 const GatSlot &fs = C.P.slots[f];
                 b_lds = in_lds<GS>(C, f);
@@ -430,8 +479,13 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
                 else if constexpr (GS) Bg = C.store + (long long)fs.store * a.cap;
                 const int flen = S->slen[f];
                 if (pass && !dup) blen = t == INT_MAX ? flen : (b_lds ? lbound(Bl, flen, t) : lbound(Bg, flen, t));
+                if (!b_lds && fs.kind == GATP_ALIAS) hub_b = uni(hub_of(a, S->val[fs.depth]));
             }
-            const bool sa = !has_r || stream_a(a, alen, blen, b_lds);
+            // candidate lists with a hub bitmap are probed through it
+            const int hub_a = (has_r && valid) ? hub_of(a, c) : -1;
+            const int lim_a = hub_a >= 0 ? list_lim(a, ab, alen) : 0;
+            if (hub_b >= 0 && pass && !dup) lim_b = blen > 0 ? Bg[blen - 1] : -1;
+            const bool sa = !has_r || stream_a(a, alen, blen, b_lds || hub_b >= 0, hub_a >= 0);
             const int mylen = sa ? alen : blen;
             int T = 0;
             const int myoff = excl_scan(mylen, &T);
@@ -448,16 +502,22 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
                 const long long s_ab = bperm64(ab, s);
                 const int s_alen = bperm(alen, s);
                 const int s_blen = bperm(blen, s);
+                const int s_hub = bperm(hub_a, s);
+                const int s_lim = bperm(lim_a, s);
+                const int s_limb = bperm(lim_b, s);
                 bool hit = false;
                 if (j < T) {
                     const int idx = j - s_off;
                     int x;
                     if (s_sa) {
                         x = a.adj[s_ab + idx];
-                        hit = !has_r || (b_lds ? has(Bl, s_blen, x, a.lin_thr) : has(Bg, s_blen, x, a.lin_thr));
+                        if (!has_r) hit = true;
+                        else if (b_lds) hit = has(Bl, s_blen, x, a.lin_thr);
+                        else if (hub_b >= 0) hit = hub_has(a, hub_b, x, s_limb);
+                        else hit = has(Bg, s_blen, x, a.lin_thr);
                     } else {
                         x = b_lds ? Bl[idx] : Bg[idx];
-                        hit = has(a.adj + s_ab, s_alen, x, a.lin_thr);
+                        hit = s_hub >= 0 ? hub_has(a, s_hub, x, s_lim) : has(a.adj + s_ab, s_alen, x, a.lin_thr);
                     }
                     for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
                 }
@@ -987,6 +1047,20 @@ __global__ void split_kernel(const long long *off, const int *adj, int n, int *s
         split[v] = lbound(adj + off[v], (int)(off[v + 1] - off[v]), (int)v);
 }
 
+// Bitmap row h of hub vertex hub_vertex[h] (or h when hub_vertex is null);
+// rows start zeroed.
+__global__ void hub_rows_kernel(const long long *off, const int *adj, const int *hub_vertex, int n_hubs,
+                                unsigned *bits, long long words) {
+    for (int h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+        const int v = hub_vertex ? hub_vertex[h] : h;
+        unsigned *row = bits + (long long)h * words;
+        for (long long k = off[v] + threadIdx.x; k < off[v + 1]; k += blockDim.x) {
+            const int x = adj[k];
+            atomicOr(&row[x >> 5], 1u << (x & 31));
+        }
+    }
+}
+
 __global__ void copy_kernel(const int4 *__restrict__ src, int4 *__restrict__ dst, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -1051,7 +1125,23 @@ struct DeviceGraph {
     long long *off = nullptr;
     int *adj = nullptr;
     int *split = nullptr;
+    // hub bitmaps, built for the threshold hub_degree (0 = not built)
+    int hub_degree = 0;
+    int n_hubs = 0;
+    long long hub_words = 0;
+    unsigned *hub_bits = nullptr;
+    int *hub_map = nullptr;     // null when the graph is degree-ordered
+    int *hub_vertex = nullptr;
+    void free_hubs() {
+        for (void *p : {(void *)hub_bits, (void *)hub_map, (void *)hub_vertex})
+            if (p) (void)hipFree(p);
+        hub_bits = nullptr;
+        hub_map = hub_vertex = nullptr;
+        n_hubs = 0;
+        hub_degree = 0;
+    }
     ~DeviceGraph() {
+        free_hubs();
         if (off) (void)hipFree(off);
         if (adj) (void)hipFree(adj);
         if (split) (void)hipFree(split);
@@ -1113,6 +1203,53 @@ static int ensure_device_graph(Graph &g, std::shared_ptr<DeviceGraph> *out) {
     return GAT_OK;
 }
 
+// Neighbourhood bitmaps for every vertex of degree >= thr, most connected
+// first, within kHubBytes. Built once per (graph, threshold) outside the
+// timed count, like the graph upload itself.
+constexpr double kHubBytes = 8.0 * (1ull << 30);
+
+static int ensure_hubs(const Graph &g, DeviceGraph *dg, int thr) {
+    if (dg->hub_degree == thr) return GAT_OK;
+    dg->free_hubs();
+    const long long words = ((long long)g.n + 31) / 32;
+    bool ordered = true;  // degree-ordered graphs keep hubs as a prefix
+    for (int32_t v = 1; v < g.n && ordered; v++) ordered = g.deg(v) <= g.deg(v - 1);
+    std::vector<int32_t> hubs;
+    if (ordered) {
+        int32_t h = 0;
+        while (h < g.n && g.deg(h) >= thr) h++;
+        hubs.resize(h);
+        std::iota(hubs.begin(), hubs.end(), 0);
+    } else {
+        for (int32_t v = 0; v < g.n; v++)
+            if (g.deg(v) >= thr) hubs.push_back(v);
+        std::stable_sort(hubs.begin(), hubs.end(), [&](int32_t x, int32_t y) { return g.deg(x) > g.deg(y); });
+    }
+    const long long max_hubs = (long long)(kHubBytes / (4.0 * std::max(1ll, words)));
+    if ((long long)hubs.size() > max_hubs) hubs.resize(max_hubs);
+    dg->hub_degree = thr;
+    dg->hub_words = words;
+    if (hubs.empty()) return GAT_OK;
+    const long long n_u32 = (long long)hubs.size() * words;
+    HIPCHECK(hipMalloc(&dg->hub_bits, sizeof(unsigned) * (size_t)((n_u32 + 1) / 2 * 2)));
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(2048), dim3(256), 0, nullptr, (unsigned long long *)dg->hub_bits,
+                       (n_u32 + 1) / 2, 0ull);
+    if (!ordered) {
+        std::vector<int32_t> map(size_t(g.n), -1);
+        for (size_t h = 0; h < hubs.size(); h++) map[hubs[h]] = int32_t(h);
+        HIPCHECK(hipMalloc(&dg->hub_map, sizeof(int) * size_t(g.n)));
+        HIPCHECK(hipMalloc(&dg->hub_vertex, sizeof(int) * hubs.size()));
+        HIPCHECK(hipMemcpy(dg->hub_map, map.data(), sizeof(int) * size_t(g.n), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dg->hub_vertex, hubs.data(), sizeof(int) * hubs.size(), hipMemcpyHostToDevice));
+    }
+    hipLaunchKernelGGL(gatk::hub_rows_kernel, dim3((unsigned)std::min<size_t>(hubs.size(), 65535)), dim3(256), 0,
+                       nullptr, dg->off, dg->adj, dg->hub_vertex, (int)hubs.size(), dg->hub_bits, words);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipDeviceSynchronize());
+    dg->n_hubs = (int)hubs.size();
+    return GAT_OK;
+}
+
 struct Launch {
     gatk::KArgs a;
     dim3 grid, block;
@@ -1139,6 +1276,7 @@ static void fill_defaults(gat_kernel_config *c) {
     if (c->intersect_strategy < 0 || c->intersect_strategy > 2) c->intersect_strategy = 0;
     if (c->gallop_ratio <= 0) c->gallop_ratio = 32;
     if (c->search_linear_threshold < 0) c->search_linear_threshold = 0;
+    if (c->hub_degree == 0) c->hub_degree = 256;
 }
 
 static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
@@ -1165,14 +1303,21 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 2));
         HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
         for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
-        // HBM buffers for the overflow launch: every set fits in max degree
-        d->ovf_grid = std::max(64, di.cus);
-        d->gstore_bytes = (size_t)d->ovf_grid * std::max(1, pd.n_store) * (size_t)std::max(1, g.max_deg) * 4;
-        HIPCHECK(hipMalloc(&d->gstore, std::max<size_t>(d->gstore_bytes, 64)));
     }
     PlanDevice *d = p->dev;
     const long long chunk = cfg.queue_chunk;
     const bool clique = p->cp.clique_ks != 0 && cfg.vertex_store != 1;
+    if (!clique && !d->gstore) {
+        // HBM set buffers of the overflow launch, where every set fits in the
+        // maximum degree: 8 waves per CU hide the latency of HBM-resident
+        // sets, within 4 GiB of buffers.
+        const size_t per_wave = (size_t)std::max(1, pd.n_store) * (size_t)std::max(1, g.max_deg) * 4;
+        int waves = std::max(64, 8 * di.cus);
+        while (waves > 64 && (double)waves * per_wave > 4.0 * (1 << 30)) waves /= 2;
+        d->ovf_grid = waves;
+        d->gstore_bytes = (size_t)waves * per_wave;
+        HIPCHECK(hipMalloc(&d->gstore, std::max<size_t>(d->gstore_bytes, 64)));
+    }
     if (cfg.vertex_store == 2 && !clique)
         GAT_FAIL(GAT_EINFEASIBLE, "bitmap vertex store needs a plan of cliques K3..K6 in unique or labeled mode");
     const long long ovf_cap = clique ? g.m + 1 : (g.m + chunk - 1) / chunk + 1;
@@ -1207,6 +1352,14 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     a.gallop_ratio = cfg.gallop_ratio;
     a.strategy = cfg.intersect_strategy;
     a.account = 0;
+    if (!clique && cfg.hub_degree > 0) {
+        rc = ensure_hubs(g, d->dg.get(), cfg.hub_degree);
+        if (rc) return rc;
+        a.hub_bits = d->dg->hub_bits;
+        a.hub_map = d->dg->hub_map;
+        a.hub_words = d->dg->hub_words;
+        a.n_hubs = d->dg->n_hubs;
+    }
     L->a = a;
     L->block = dim3(64 * cfg.block_waves);
     const int total_waves = std::max(1, cfg.waves_per_cu * di.cus);

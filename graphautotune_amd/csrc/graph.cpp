@@ -1,6 +1,7 @@
 #include "graph.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -304,6 +305,48 @@ void relabel_by_degree(const Graph &in, Graph *out, std::vector<int32_t> *old_of
     });
     out->finish();
     if (old_of_new) *old_of_new = std::move(order);
+}
+
+int64_t count_triangles(const Graph &g) {
+    // below[v] = number of neighbours of v with a smaller id (rows are sorted)
+    std::vector<int32_t> below(size_t(g.n), 0);
+    parallel_for(g.n, [&](int64_t lo, int64_t hi) {
+        for (int64_t v = lo; v < hi; v++) {
+            const auto b = g.adj.begin() + g.off[v], e = g.adj.begin() + g.off[v + 1];
+            below[v] = int32_t(std::lower_bound(b, e, int32_t(v)) - b);
+        }
+    });
+    std::atomic<int64_t> next{0}, total{0};
+    const int64_t block = 2048;
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&] {
+        int64_t local = 0;
+        for (;;) {
+            const int64_t lo = next.fetch_add(block);
+            if (lo >= g.n) break;
+            const int64_t hi = std::min<int64_t>(g.n, lo + block);
+            for (int64_t v = lo; v < hi; v++) {
+                const int32_t *a = g.adj.data() + g.off[v];
+                const int32_t na = below[v];
+                for (int32_t k = 0; k < na; k++) {
+                    const int32_t u = a[k];
+                    const int32_t *b = g.adj.data() + g.off[u];
+                    const int32_t nb = below[u];
+                    int32_t i = 0, j = 0;
+                    while (i < na && j < nb) {  // common neighbours w < u < v
+                        if (a[i] < b[j]) i++;
+                        else if (a[i] > b[j]) j++;
+                        else { local++; i++; j++; }
+                    }
+                }
+            }
+        }
+        total += local;
+    };
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < hw; t++) ts.emplace_back(work);
+    for (auto &t : ts) t.join();
+    return total.load();
 }
 
 }  // namespace gat

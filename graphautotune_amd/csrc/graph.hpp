@@ -49,4 +49,9 @@ int gen_chung_lu(int32_t n, double mean_deg, double gamma, int32_t max_deg, uint
 // id. old_of_new may be null.
 void relabel_by_degree(const Graph &in, Graph *out, std::vector<int32_t> *old_of_new);
 
+// Number of triangles w < u < v: for each edge (u < v), the common
+// neighbours below u, by merging the two rows' prefixes of smaller ids
+// (cheap on a degree-ordered graph, where those prefixes are short).
+int64_t count_triangles(const Graph &g);
+
 }  // namespace gat

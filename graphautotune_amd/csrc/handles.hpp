@@ -14,6 +14,7 @@ void destroy_plan_device(PlanDevice *);
 struct gat_graph {
     std::shared_ptr<gat::Graph> g;
     std::shared_ptr<gat::Graph> relabeled;  // lazily built degree order
+    int64_t triangles = -1;                 // lazily counted
     std::mutex mu;
 };
 

@@ -343,33 +343,30 @@ struct Builder {
 
 }  // namespace
 
-int build_schedule(const Pattern &p, Schedule *out) {
+static int check_pattern(const Pattern &p) {
     if (p.n < 3) GAT_FAIL(GAT_EUNSUPPORTED, "patterns need at least 3 vertices (got %d)", p.n);
     if (p.n > 8)
         GAT_FAIL(GAT_ELIMIT, "schedule search is O(n!^2); patterns above 8 vertices are refused");
-    {  // connectivity: some order must be valid
-        uint32_t seen = 1, frontier = 1;
-        while (frontier) {
-            uint32_t nxt = 0;
-            for (int v = 0; v < p.n; v++)
-                if (frontier >> v & 1) nxt |= p.row[v];
-            frontier = nxt & ~seen;
-            seen |= nxt;
-        }
-        if (seen != (1u << p.n) - 1) GAT_FAIL(GAT_EINVAL, "pattern is not connected");
+    uint32_t seen = 1, frontier = 1;  // connectivity: some order must be valid
+    while (frontier) {
+        uint32_t nxt = 0;
+        for (int v = 0; v < p.n; v++)
+            if (frontier >> v & 1) nxt |= p.row[v];
+        frontier = nxt & ~seen;
+        seen |= nxt;
     }
-    Schedule s;
-    s.input = p;
-    s.n = p.n;
-    const std::vector<Perm> aut = automorphisms(p);
-    s.automorphism_count = (int)aut.size();
+    if (seen != (1u << p.n) - 1) GAT_FAIL(GAT_EINVAL, "pattern is not connected");
+    return GAT_OK;
+}
 
-    // ---- permutation search (schedule.hpp:864-926) ----
-    Perm best_perm;
-    std::vector<Pair> best_r;
+int schedule_candidates(const Pattern &p, const ScheduleStats &st, std::vector<ScheduleCandidate> *out) {
+    int rc = check_pattern(p);
+    if (rc) return rc;
+    const std::vector<Perm> aut = automorphisms(p);
+    // ---- permutation search (schedule.hpp:864-926): every (permutation,
+    // restriction set) at the largest IEP suffix, in scan order ----
+    std::vector<ScheduleCandidate> all;
     int best_suffix = -1;
-    double best_cost = 1e18;
-    bool have_best = false;
     Perm perm(p.n);
     std::iota(perm.begin(), perm.end(), 0);
     do {
@@ -379,8 +376,7 @@ int build_schedule(const Pattern &p, Schedule *out) {
         if (suf < best_suffix) continue;
         if (suf > best_suffix) {
             best_suffix = suf;
-            best_cost = 1e18;
-            have_best = false;
+            all.clear();
         }
         // Aut(q) = perm o Aut(p) o perm^-1, listed in lexicographic order
         // as a direct enumeration of Aut(q) would produce it.
@@ -391,20 +387,45 @@ int build_schedule(const Pattern &p, Schedule *out) {
             autq.push_back(std::move(t));
         }
         std::sort(autq.begin(), autq.end());
-        for (const auto &r : restriction_candidates_from(q, autq)) {
-            const double est = estimate_restrictions(q, r, 5000, 1000000, 10000);
-            if (!have_best || est < best_cost) {
-                have_best = true;
-                best_cost = est;
-                best_perm = perm;
-                best_r = r;
-            }
-        }
+        for (const auto &r : restriction_candidates_from(q, autq))
+            all.push_back(ScheduleCandidate{perm, r, estimate_restrictions(q, r, st.v_cnt, st.e_cnt, st.tri_cnt)});
     } while (std::next_permutation(perm.begin(), perm.end()));
-    if (!have_best) GAT_FAIL(GAT_EINVAL, "no valid schedule for pattern %s", p.str().c_str());
+    if (all.empty()) GAT_FAIL(GAT_EINVAL, "no valid schedule for pattern %s", p.str().c_str());
+    // The reference keeps the first candidate of strictly smallest cost, so
+    // a stable sort puts its pick first. Candidates that relabel the pattern
+    // identically and carry the same restriction set compile to the same
+    // plan: only the first of them is kept.
+    std::stable_sort(all.begin(), all.end(),
+                     [](const ScheduleCandidate &x, const ScheduleCandidate &y) { return x.cost < y.cost; });
+    std::set<std::pair<std::vector<uint16_t>, std::vector<Pair>>> seen;
+    out->clear();
+    for (auto &c : all) {
+        const Pattern q = p.relabel(c.perm.data());
+        std::vector<Pair> r = c.restrictions;
+        std::sort(r.begin(), r.end());
+        if (seen.insert({std::vector<uint16_t>(q.row, q.row + q.n), r}).second) out->push_back(std::move(c));
+    }
+    return GAT_OK;
+}
 
+int build_schedule(const Pattern &p, Schedule *out) {
+    std::vector<ScheduleCandidate> c;
+    int rc = schedule_candidates(p, ScheduleStats{}, &c);
+    if (rc) return rc;
+    return build_schedule_from(p, c[0], out);
+}
+
+int build_schedule_from(const Pattern &p, const ScheduleCandidate &cand, Schedule *out) {
+    int rc = check_pattern(p);
+    if (rc) return rc;
+    Schedule s;
+    s.input = p;
+    s.n = p.n;
+    s.automorphism_count = (int)automorphisms(p).size();
+    const Perm &best_perm = cand.perm;
+    const std::vector<Pair> &best_r = cand.restrictions;
     for (int i = 0; i < p.n; i++) s.perm[i] = best_perm[i];
-    s.best_cost = best_cost;
+    s.best_cost = cand.cost;
     const Pattern q = p.relabel(best_perm.data());
     s.ordered = q;
     s.chosen = best_r;

@@ -106,8 +106,31 @@ struct Schedule {
     double best_cost = 0;
 };
 
+// Graph statistics of the cost estimate. The reference's Schedule always
+// passes these fixed numbers (schedule.hpp:909-910).
+struct ScheduleStats {
+    int v_cnt = 5000;
+    int64_t e_cnt = 1000000;
+    int64_t tri_cnt = 10000;
+};
+
+struct ScheduleCandidate {
+    std::vector<int> perm;           // vertex i of the input becomes perm[i]
+    std::vector<Pair> restrictions;  // on the relabelled pattern
+    double cost = 0;                 // estimate_restrictions under the stats
+};
+
+// Every (permutation, restriction set) the reference's search considers at
+// the largest IEP suffix (schedule.hpp:878-926), ranked by the estimate under
+// `st` (stable, so with the fixed stats entry 0 is the reference's pick), with
+// duplicates that would compile to the same plan removed.
+int schedule_candidates(const Pattern &p, const ScheduleStats &st, std::vector<ScheduleCandidate> *out);
+
 // Core::Schedule(const Pattern&) (schedule.hpp:864-958).
 int build_schedule(const Pattern &p, Schedule *out);
+// The rest of the Schedule constructor (schedule.hpp:928-958) for a given
+// permutation and restriction set.
+int build_schedule_from(const Pattern &p, const ScheduleCandidate &c, Schedule *out);
 
 void export_schedule(const Schedule &s, gat_schedule_data *out);
 

@@ -242,10 +242,15 @@ struct Compiler {
 
 }  // namespace
 
-int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *out) {
+int compile_plan(const std::vector<std::string> &adj, int mode, const PlanOptions &opt, CompiledPlan *out) {
     if (adj.empty()) GAT_FAIL(GAT_EINVAL, "no patterns");
     if (mode != GAT_MODE_UNIQUE && mode != GAT_MODE_REFERENCE && mode != GAT_MODE_LABELED)
         GAT_FAIL(GAT_EINVAL, "unknown count mode %d", mode);
+    if (opt.schedule_policy != GAT_SCHEDULE_REFERENCE && opt.schedule_policy != GAT_SCHEDULE_GRAPH)
+        GAT_FAIL(GAT_EINVAL, "unknown schedule policy %d", opt.schedule_policy);
+    if (opt.schedule_policy == GAT_SCHEDULE_GRAPH && mode == GAT_MODE_REFERENCE)
+        GAT_FAIL(GAT_EINVAL, "reference-mode counts are defined only for the reference's schedule");
+    if (opt.schedule_rank < 0) GAT_FAIL(GAT_EINVAL, "negative schedule rank");
     if ((int)adj.size() > GATP_MAX_TERMS) GAT_FAIL(GAT_ELIMIT, "at most %d patterns per plan", GATP_MAX_TERMS);
     CompiledPlan cp;
     cp.relabel = mode != GAT_MODE_REFERENCE;
@@ -257,7 +262,15 @@ int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *ou
         Pattern p;
         int rc = parse_pattern(adj[pi], &p);
         if (rc) return rc;
-        rc = build_schedule(p, &info.sched);
+        if (opt.schedule_policy == GAT_SCHEDULE_REFERENCE) {
+            rc = build_schedule(p, &info.sched);
+        } else {
+            std::vector<ScheduleCandidate> cands;
+            rc = schedule_candidates(p, opt.stats, &cands);
+            if (!rc)
+                rc = build_schedule_from(p, cands[std::min<size_t>((size_t)opt.schedule_rank, cands.size() - 1)],
+                                         &info.sched);
+        }
         if (rc) return rc;
         const Schedule &s = info.sched;
         const Pattern &q = s.ordered;

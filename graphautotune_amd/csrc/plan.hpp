@@ -37,8 +37,18 @@ struct CompiledPlan {
 // automorphic embeddings. Returns c (>= 1).
 uint64_t unique_restrictions(const Schedule &s, std::vector<Pair> *used);
 
+// How each pattern's Schedule is chosen. REFERENCE: the reference's pick
+// (the only choice whose reference-mode count is defined). GRAPH (unique and
+// labeled modes): the candidates ranked by the cost estimate under `stats`,
+// taking entry `rank` (clamped to the last one).
+struct PlanOptions {
+    int schedule_policy = GAT_SCHEDULE_REFERENCE;
+    int schedule_rank = 0;
+    ScheduleStats stats;
+};
+
 // mode: GAT_MODE_UNIQUE / GAT_MODE_REFERENCE / GAT_MODE_LABELED
-int compile_plan(const std::vector<std::string> &adj, int mode, CompiledPlan *out);
+int compile_plan(const std::vector<std::string> &adj, int mode, const PlanOptions &opt, CompiledPlan *out);
 
 std::string describe_plan(const GatPlanDev &d);
 

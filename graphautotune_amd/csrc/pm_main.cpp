@@ -38,7 +38,8 @@ static bool set_knob(gat_kernel_config *c, const std::string &k, int v) {
                  {"gallop_ratio", &c->gallop_ratio},
                  {"search_linear_threshold", &c->search_linear_threshold},
                  {"max_regs", &c->max_regs},
-                 {"vertex_store", &c->vertex_store}};
+                 {"vertex_store", &c->vertex_store},
+                 {"hub_degree", &c->hub_degree}};
     for (auto &kn : knobs)
         if (k == kn.name) {
             *kn.field = v;

@@ -58,6 +58,13 @@ extern "C" {
 #define GAT_MODE_REFERENCE 1
 #define GAT_MODE_LABELED 2
 
+/* How a plan picks each pattern's Schedule. */
+#define GAT_SCHEDULE_REFERENCE 0 /* the reference's pick: cost estimate under its
+                                    fixed stats (5000, 1e6, 1e4), schedule.hpp:909-910 */
+#define GAT_SCHEDULE_GRAPH 1     /* unique/labeled only: candidates ranked by the
+                                    same estimate under this graph's vertex,
+                                    directed-edge and triangle counts */
+
 /* ---- opaque handles -------------------------------------------------- */
 typedef struct gat_graph gat_graph; /* host CSR (+ device copy once used) */
 typedef struct gat_plan gat_plan;   /* compiled schedules + device state   */
@@ -117,7 +124,11 @@ typedef struct gat_kernel_config {
     int32_t vertex_store;     /* 0 auto, 1 sorted arrays only, 2 bitmap rows for
                                  clique plans (VertexStoreType {Array, Bitmap},
                                  vertex_set_config.hpp:3-6)                    */
-    int32_t reserved[2];
+    int32_t hub_degree;       /* array store: vertices of at least this degree
+                                 get a dense neighbourhood bitmap in HBM that
+                                 replaces binary search in their lists
+                                 (0 = 256, negative = none)                   */
+    int32_t reserved;
 } gat_kernel_config;
 
 /* Per-call statistics filled by gat_count (any pointer may be NULL). */
@@ -136,6 +147,14 @@ typedef struct gat_count_stats {
 
 #define GAT_FLAG_ACCOUNT 1u   /* instrumented pass: accumulate W_alg bytes  */
 
+/* Plan options (gat_plan_create_ex). Zero-initialised = gat_plan_create. */
+typedef struct gat_plan_options {
+    int32_t schedule_policy;  /* GAT_SCHEDULE_*                              */
+    int32_t schedule_rank;    /* GRAPH: take the rank-th candidate (0 = best),
+                                 clamped to the last one                      */
+    int64_t reserved[3];
+} gat_plan_options;
+
 /* ---- errors ---------------------------------------------------------- */
 const char *gat_last_error(void);
 const char *gat_version(void);
@@ -146,6 +165,16 @@ const char *gat_version(void);
  * the ScheduleData record (schedule.hpp:984-1026). `out` must point to a
  * gat_schedule_data; `cap` is its size in bytes. */
 int gat_schedule_build(const char *adj_str, gat_schedule_data *out, size_t cap);
+
+/* Every (permutation, restriction set) Schedule candidate of the reference's
+ * search at the largest IEP suffix (schedule.hpp:878-926), ranked by
+ * estimate_restrictions (schedule.hpp:362-437) under the given graph
+ * statistics, stably (with v_cnt=5000, e_cnt=1e6, tri_cnt=1e4 entry 0 is
+ * gat_schedule_build's schedule), without candidates that compile to the same
+ * plan. Writes up to `cap` records of `elem_size` bytes each to `out` and the
+ * total number of candidates to *n_out. */
+int gat_schedule_candidates(const char *adj_str, int32_t v_cnt, int64_t e_cnt, int64_t tri_cnt,
+                            gat_schedule_data *out, size_t elem_size, int32_t cap, int32_t *n_out);
 
 /* |Aut(P)| of a pattern string (schedule.hpp:189-191). */
 int gat_pattern_automorphisms(const char *adj_str, int64_t *out);
@@ -183,6 +212,8 @@ int gat_graph_csr(const gat_graph *g, const int64_t **offsets, const int32_t **a
 /* 32-bin log2-degree histogram (bin k counts vertices with
  * floor(log2(deg+1)) == k), used as cost-model features. */
 int gat_graph_degree_histogram(const gat_graph *g, int64_t *bins32);
+/* Number of triangles (host count on the degree-ordered graph, cached). */
+int gat_graph_triangles(gat_graph *g, int64_t *out);
 void gat_graph_free(gat_graph *g);
 
 /* ---- plans and counting --------------------------------------------- */
@@ -193,6 +224,11 @@ void gat_graph_free(gat_graph *g);
  * device buffers are created on the first gat_count. */
 int gat_plan_create(gat_graph *g, const char *const *adj_strs, int32_t n_patterns,
                     int32_t mode, gat_plan **out);
+/* gat_plan_create with options (NULL = defaults). GAT_SCHEDULE_GRAPH picks
+ * schedules with this graph's statistics: the reference's Schedule hard-codes
+ * its cost-model inputs (schedule.hpp:909-910). */
+int gat_plan_create_ex(gat_graph *g, const char *const *adj_strs, int32_t n_patterns,
+                       int32_t mode, const gat_plan_options *opt, gat_plan **out);
 /* Number of patterns and the schedule record of pattern i. */
 int gat_plan_schedule(const gat_plan *p, int32_t i, gat_schedule_data *out, size_t cap);
 /* Replaces Executor::perform_search (engine.cuh:232-265) for every pattern

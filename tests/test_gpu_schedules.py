@@ -1,0 +1,35 @@
+"""Graph-aware Schedule choice (GPU): every candidate Schedule of a pattern
+counts the same instances, so unique/labeled counts must not depend on the
+rank picked; checked against the oracle."""
+import pytest
+
+from patterns import C5, HOUSE, connected_patterns
+
+pytestmark = pytest.mark.gpu
+
+
+def test_every_candidate_counts_the_oracle_value(gat, orc, gpu):
+    g = gat.Graph.chung_lu(200, 8.0, 2.1, 30, 5)
+    off, adj = g.csr()
+    stats = (g.v_cnt(), 2 * g.undirected_edges(), g.triangles())
+    pats = [HOUSE, C5] + connected_patterns(5)[1::5] + connected_patterns(6)[7::37]
+    for p in pats:
+        want = orc.unique_count(off, adj, p)
+        _, total = gat.schedule_candidates(p, *stats, limit=1)
+        ranks = sorted(set(range(min(total, 6))) | {total - 1})
+        for r in ranks:
+            plan = gat.Plan(g, [p], "unique", schedule="graph", schedule_rank=r)
+            assert plan.count()[0] == [want], (p, r)
+            assert plan.count({"slot_capacity": 16})[0] == [want], (p, r, "overflow")
+        aut = gat.Pattern(p).automorphisms()
+        lab = gat.Plan(g, [p], "labeled", schedule="graph", schedule_rank=ranks[-1])
+        assert lab.count()[0] == [aut * want], p
+
+
+def test_batched_graph_schedules(gat, orc, gpu):
+    g = gat.Graph.chung_lu(300, 8.0, 2.1, 40, 5)
+    off, adj = g.csr()
+    pats = connected_patterns(5)
+    want = [orc.unique_count(off, adj, p) for p in pats]
+    for r in (0, 1, 3):
+        assert gat.Plan(g, pats, "unique", schedule="graph", schedule_rank=r).count()[0] == want, r

@@ -111,3 +111,16 @@ def test_degree_histogram(gat):
     deg = np.diff(g.csr()[0])
     want = np.bincount(np.floor(np.log2(deg + 1)).astype(int), minlength=32)[:32]
     assert np.array_equal(h, want) and h.sum() == 300
+
+
+def test_triangle_count(gat, orc):
+    from patterns import TRIANGLE
+    for g in (gat.Graph.erdos_renyi(300, 0.1, 5), gat.Graph.rmat(9, 8, 0.57, 0.19, 0.19, 4),
+              gat.Graph.chung_lu(400, 10.0, 2.1, 60, 2)):
+        off, adj = g.csr()
+        assert g.triangles() == orc.unique_count(off, adj, TRIANGLE)
+    m = 23  # K_m: C(m,3)
+    src = [i for i in range(m) for j in range(i + 1, m)]
+    dst = [j for i in range(m) for j in range(i + 1, m)]
+    assert gat.Graph.from_edges(m, src, dst).triangles() == m * (m - 1) * (m - 2) // 6
+    assert gat.Graph.from_edges(4, [0, 1, 2], [1, 2, 3]).triangles() == 0

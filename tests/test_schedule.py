@@ -129,3 +129,64 @@ def test_bad_patterns_rejected(gat):
     for bad in ("01", "0110x", "0" * 5, "010100000", "0100100000000000"):  # not square / chars / disconnected
         with pytest.raises((GatError, ValueError)):
             gat.Schedule(bad)
+
+
+# ---- Schedule candidates (graph-aware choice for unique/labeled plans) ----
+
+def _restrictions(s):
+    d = s.data
+    return sorted((d.all_restr_first[i], d.all_restr_second[i]) for i in range(d.n_all_restrictions))
+
+
+def test_candidate_zero_is_the_reference_schedule(gat):
+    for p in all_patterns():
+        cands, total = gat.schedule_candidates(p, limit=1)
+        assert total >= 1 and bytes(cands[0].data) == bytes(gat.Schedule(p).data), p
+
+
+@pytest.mark.parametrize("p", [HOUSE, C5, K4] + connected_patterns(5)[::4] + connected_patterns(6)[::23])
+def test_every_candidate_is_a_valid_schedule(gat, p):
+    """Each candidate relabels the pattern into a valid order with the largest
+    IEP suffix, carries a restriction set that leaves exactly n!/|Aut|
+    orderings (the validity test of schedule.hpp:346-357, by brute force),
+    and the list is sorted by cost with no two candidates alike."""
+    ref = gat.Schedule(p)
+    n = ref.data.v_cnt
+    aut = gat.Pattern(p).automorphisms()
+    for stats in ((5000, 10 ** 6, 10 ** 4), (65536, 1819096, 2855038)):
+        cands, total = gat.schedule_candidates(p, *stats, limit=512)
+        assert len(cands) == total
+        seen = set()
+        for k, s in enumerate(cands):
+            d = s.data
+            assert d.iep_suffix_vertexes == ref.data.iep_suffix_vertexes
+            rows = tuple(d.adj_rows[i] for i in range(n))
+            for i in range(1, n):  # every vertex after the first has an earlier neighbour
+                assert rows[i] & ((1 << i) - 1)
+            # the ordered pattern is the input relabelled by perm
+            pin = gat.Pattern(p)
+            for i in range(n):
+                for j in range(n):
+                    assert pin.has_edge(i, j) == bool(rows[d.perm[i]] >> d.perm[j] & 1)
+            R = _restrictions(s)
+            assert n_linear_extensions(n, R) == math.factorial(n) // aut
+            key = (rows, tuple(R))
+            assert key not in seen
+            seen.add(key)
+            if k:
+                assert cands[k - 1].data.best_cost <= d.best_cost
+
+
+def test_graph_schedule_policy_rules(gat):
+    from graphautotune_amd._lib import GAT_EINVAL, GatError
+    g = gat.Graph.erdos_renyi(40, 0.3, 3)
+    with pytest.raises(GatError) as e:
+        gat.Plan(g, [HOUSE], "reference", schedule="graph")
+    assert e.value.code == GAT_EINVAL
+    with pytest.raises(GatError):
+        gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=-1)
+    # a rank past the end takes the last candidate
+    _, total = gat.schedule_candidates(HOUSE, g.v_cnt(), 2 * g.undirected_edges(), g.triangles())
+    last = gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=total - 1).schedule(0)
+    past = gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=total + 5).schedule(0)
+    assert bytes(last.data) == bytes(past.data)
