@@ -752,6 +752,103 @@ __device__ __forceinline__ int clique_isect(AP A, int alen, BP B, int blen, int 
     return total;
 }
 
+// Bit rows of the subgraph induced on S (LDS, s <= 32 * MW entries):
+// row_i = { j < i : S[j] in N+(S[i]) }, built by one flattened sweep over
+// the lists N+(S[i]); then #K4 += sum_i popc(row_i), #K5 += sum over set
+// bits j of row_i of popc(row_i & row_j), and #K6 one level deeper.
+template <int MW>
+__device__ void clique_rows_count(const CArgs &a, const int *sv, int s, unsigned *rows, int ln,
+                                  unsigned long long &acc4, unsigned long long &acc5, unsigned long long &acc6,
+                                  unsigned long long &bytes) {
+    // ---- bit rows of the subgraph induced on S ----
+    const int w = (s + 31) >> 5;
+    for (int k = ln; k < s * w; k += W) rows[k] = 0u;
+    wsync();
+    for (int cb = 0; cb < s; cb += W) {
+        const int ci = cb + ln;
+        const bool valid = ci < s;
+        const int x = valid ? sv[ci] : 0;
+        const long long xb = valid ? a.off[x] : 0;
+        const int xl = valid ? a.split[x] : 0;  // N+(x): all below x
+        // stream N+(x) and search S[0,ci), or the reverse when skewed
+        const bool sx = (long long)xl <= (long long)a.gallop_ratio * (long long)ci;
+        const int mylen = valid ? (sx ? xl : ci) : 0;
+        if (a.account && valid) bytes += 4ull * (xl + ci) + 16ull;
+        int T = 0;
+        const int myoff = excl_scan(mylen, &T);
+        for (int k0 = 0; k0 < T; k0 += W) {
+            const int j = k0 + ln;
+            int so = 0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1)
+                if (bperm(myoff, so + step) <= j && so + step < W) so += step;
+            // every lane takes part in the cross-lane reads: a
+            // segment's owner lane may itself have no element here
+            const int s_off = bperm(myoff, so);
+            const int s_sx = bperm((int)sx, so);
+            const long long s_xb = bperm64(xb, so);
+            const int s_xl = bperm(xl, so);
+            if (j < T) {
+                const int s_ci = cb + so;
+                const int idx = j - s_off;
+                int pos = -1;
+                if (s_sx) {
+                    const int y = a.adj[s_xb + idx];
+                    const int p = lbound(sv, s_ci, y);
+                    if (p < s_ci && sv[p] == y) pos = p;
+                } else {
+                    if (has(a.adj + s_xb, s_xl, sv[idx], a.lin_thr)) pos = idx;
+                }
+                if (pos >= 0) atomicOr(&rows[s_ci * w + (pos >> 5)], 1u << (pos & 31));
+            }
+        }
+    }
+    wsync();
+    // ---- counts from the rows ----
+    for (int k = ln; k < s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
+    if (a.account && ln == 0) bytes += 4ull * s * w;
+    if (a.kmax >= 5) {
+        for (int ib = 0; ib < s; ib += W) {
+            const int ii = ib + ln;
+            if (ii >= s) continue;
+            unsigned ri[MW];
+#pragma unroll
+            for (int q = 0; q < MW; q++) ri[q] = q < w ? rows[ii * w + q] : 0u;
+            for (int q = 0; q < MW; q++) {
+                if (q >= w) break;
+                unsigned bits = ri[q];
+                while (bits) {
+                    const int jj = q * 32 + __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    unsigned rm[MW];
+                    unsigned long long c5 = 0;
+#pragma unroll
+                    for (int t = 0; t < MW; t++) {
+                        rm[t] = t < w ? (ri[t] & rows[jj * w + t]) : 0u;
+                        c5 += (unsigned long long)__popc(rm[t]);
+                    }
+                    acc5 += c5;
+                    if (a.account) bytes += 8ull * w;
+                    if (a.kmax >= 6) {
+                        for (int t = 0; t < MW; t++) {
+                            if (t >= w) break;
+                            unsigned b2 = rm[t];
+                            while (b2) {
+                                const int ll = t * 32 + __ffs(b2) - 1;
+                                b2 &= b2 - 1;
+#pragma unroll
+                                for (int u = 0; u < MW; u++)
+                                    if (u < w) acc6 += (unsigned long long)__popc(rm[u] & rows[ll * w + u]);
+                                if (a.account) bytes += 8ull * w;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
@@ -810,93 +907,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                     if (ln == 0) acc3 += (unsigned long long)s;
                     if (a.kmax < 4 || s < 2) continue;
                     wsync();
-                    // ---- bit rows of the subgraph induced on S ----
-                    const int w = (s + 31) >> 5;
-                    for (int k = ln; k < s * w; k += W) rows[k] = 0u;
-                    wsync();
-                    for (int cb = 0; cb < s; cb += W) {
-                        const int ci = cb + ln;
-                        const bool valid = ci < s;
-                        const int x = valid ? sv[ci] : 0;
-                        const long long xb = valid ? a.off[x] : 0;
-                        const int xl = valid ? a.split[x] : 0;  // N+(x): all below x
-                        // stream N+(x) and search S[0,ci), or the reverse when skewed
-                        const bool sx = (long long)xl <= (long long)a.gallop_ratio * (long long)ci;
-                        const int mylen = valid ? (sx ? xl : ci) : 0;
-                        if (a.account && valid) bytes += 4ull * (xl + ci) + 16ull;
-                        int T = 0;
-                        const int myoff = excl_scan(mylen, &T);
-                        for (int k0 = 0; k0 < T; k0 += W) {
-                            const int j = k0 + ln;
-                            int so = 0;
-#pragma unroll
-                            for (int step = 32; step >= 1; step >>= 1)
-                                if (bperm(myoff, so + step) <= j && so + step < W) so += step;
-                            // every lane takes part in the cross-lane reads: a
-                            // segment's owner lane may itself have no element here
-                            const int s_off = bperm(myoff, so);
-                            const int s_sx = bperm((int)sx, so);
-                            const long long s_xb = bperm64(xb, so);
-                            const int s_xl = bperm(xl, so);
-                            if (j < T) {
-                                const int s_ci = cb + so;
-                                const int idx = j - s_off;
-                                int pos = -1;
-                                if (s_sx) {
-                                    const int y = a.adj[s_xb + idx];
-                                    const int p = lbound(sv, s_ci, y);
-                                    if (p < s_ci && sv[p] == y) pos = p;
-                                } else {
-                                    if (has(a.adj + s_xb, s_xl, sv[idx], a.lin_thr)) pos = idx;
-                                }
-                                if (pos >= 0) atomicOr(&rows[s_ci * w + (pos >> 5)], 1u << (pos & 31));
-                            }
-                        }
-                    }
-                    wsync();
-                    // ---- counts from the rows ----
-                    for (int k = ln; k < s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
-                    if (a.account && ln == 0) bytes += 4ull * s * w;
-                    if (a.kmax >= 5) {
-                        for (int ib = 0; ib < s; ib += W) {
-                            const int ii = ib + ln;
-                            if (ii >= s) continue;
-                            unsigned ri[CMAXW];
-#pragma unroll
-                            for (int q = 0; q < CMAXW; q++) ri[q] = q < w ? rows[ii * w + q] : 0u;
-                            for (int q = 0; q < CMAXW; q++) {
-                                if (q >= w) break;
-                                unsigned bits = ri[q];
-                                while (bits) {
-                                    const int jj = q * 32 + __ffs(bits) - 1;
-                                    bits &= bits - 1;
-                                    unsigned rm[CMAXW];
-                                    unsigned long long c5 = 0;
-#pragma unroll
-                                    for (int t = 0; t < CMAXW; t++) {
-                                        rm[t] = t < w ? (ri[t] & rows[jj * w + t]) : 0u;
-                                        c5 += (unsigned long long)__popc(rm[t]);
-                                    }
-                                    acc5 += c5;
-                                    if (a.account) bytes += 8ull * w;
-                                    if (a.kmax >= 6) {
-                                        for (int t = 0; t < CMAXW; t++) {
-                                            if (t >= w) break;
-                                            unsigned b2 = rm[t];
-                                            while (b2) {
-                                                const int ll = t * 32 + __ffs(b2) - 1;
-                                                b2 &= b2 - 1;
-#pragma unroll
-                                                for (int u = 0; u < CMAXW; u++)
-                                                    if (u < w) acc6 += (unsigned long long)__popc(rm[u] & rows[ll * w + u]);
-                                                if (a.account) bytes += 8ull * w;
-                                            }
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                    }
+                    clique_rows_count<CMAXW>(a, sv, s, rows, ln, acc4, acc5, acc6, bytes);
                     wsync();
                 }
             }
@@ -916,7 +927,71 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     }
 }
 
-// Edges whose S outgrows the LDS rows: one edge per work item, S in LDS
+// Second tier: edges whose S outgrew the main kernel's rows but has at most
+// 32 * CMIDW entries keep S and the rows in LDS (one wave per workgroup,
+// fewer waves per CU). Larger ones go on to the HBM-row kernel.
+constexpr int CMIDW = 16;  // |S| <= 512
+
+struct CMidArgs {
+    CArgs c;
+    long long *ovf2;             // edges for the HBM-row kernel
+    unsigned long long *ovf2_n;
+};
+
+__global__ void __launch_bounds__(64) clique_mid_kernel(CMidArgs m) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CArgs &a = m.c;
+    const int ln = threadIdx.x % W;
+    constexpr int cap = 32 * CMIDW;
+    int *sv = reinterpret_cast<int *>(smem);
+    unsigned *rows = reinterpret_cast<unsigned *>(sv + cap);
+    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;
+    for (;;) {
+        unsigned long long r = 0;
+        if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
+        r = (unsigned long long)uni64((long long)__shfl(r, 0));
+        if (r >= *a.ovf_n || (long long)r >= a.ovf_cap) break;
+        const long long ee = a.ovf[r];
+        int lo = 0, hi = a.n;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.off[mid] <= ee) lo = mid; else hi = mid;
+        }
+        const int v0 = uni(lo);
+        const long long rb = a.off[v0];
+        const int i = (int)(ee - rb);
+        const int v1 = uni(a.adj[ee]);
+        const long long ab = a.off[v1];
+        const int alen = a.split[v1];
+        const int s = uni(clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, cap, a.lin_thr, ln));
+        if (s > cap) {
+            if (ln == 0) {
+                const unsigned long long idx = atomicAdd(m.ovf2_n, 1ull);
+                if ((long long)idx < a.ovf_cap) m.ovf2[idx] = ee; else atomicOr(a.err, 16u);
+            }
+            continue;
+        }
+        if (a.account && ln == 0) bytes += 4ull * (alen + i) + 4ull * s + 32ull;
+        if (ln == 0) acc3 += (unsigned long long)s;
+        if (s < 2) continue;
+        wsync();
+        clique_rows_count<CMIDW>(a, sv, s, rows, ln, acc4, acc5, acc6, bytes);
+        wsync();
+    }
+    acc3 = wave_sum64(acc3);
+    acc4 = wave_sum64(acc4);
+    acc5 = wave_sum64(acc5);
+    acc6 = wave_sum64(acc6);
+    bytes = wave_sum64(bytes);
+    if (ln == 0) {
+        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
+        for (int k = 3; k <= 6; k++)
+            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
+        if (a.account && bytes) atomicAdd(a.bytes, bytes);
+    }
+}
+
+// Edges whose S outgrows both LDS tiers: one edge per work item, S in LDS
 // (or HBM when the maximum out-degree is too large for LDS), bit rows in a
 // per-wave HBM/L2 scratch area of capacity cap_s x ceil(cap_s/32) words.
 struct CBigArgs {
@@ -925,6 +1000,8 @@ struct CBigArgs {
     int *gsv;         // per-wave S when not in LDS
     int cap_s;        // >= maximum out-degree
     int sv_lds;       // S staged in LDS
+    const long long *ovf2;  // edges left by the LDS tiers
+    const unsigned long long *ovf2_n;
 };
 
 __global__ void __launch_bounds__(64) clique_big_kernel(CBigArgs b) {
@@ -938,10 +1015,10 @@ __global__ void __launch_bounds__(64) clique_big_kernel(CBigArgs b) {
     unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;
     for (;;) {
         unsigned long long r = 0;
-        if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
+        if (ln == 0) r = atomicAdd(&a.queue[2], 1ull);
         r = (unsigned long long)uni64((long long)__shfl(r, 0));
-        if (r >= *a.ovf_n || (long long)r >= a.ovf_cap) break;
-        const long long ee = a.ovf[r];
+        if (r >= *b.ovf2_n || (long long)r >= a.ovf_cap) break;
+        const long long ee = b.ovf2[r];
         int lo = 0, hi = a.n;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
@@ -1083,6 +1160,7 @@ struct DeviceInfo {
     bool ok = false;
     int cus = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
     char arch[64] = {};
 };
 
@@ -1105,6 +1183,7 @@ static int device_info(DeviceInfo *out) {
         }
         info.cus = p.multiProcessorCount;
         info.lds_per_block = p.sharedMemPerBlock;
+        info.lds_per_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : p.sharedMemPerBlock;
         std::snprintf(info.arch, sizeof(info.arch), "%s", p.gcnArchName);
         if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
             set_error("libgat is built for gfx950; device is %s", p.gcnArchName);
@@ -1154,6 +1233,7 @@ struct PlanDevice {
     unsigned long long *counts = nullptr, *queue = nullptr, *ovf_n = nullptr, *bytes = nullptr;
     unsigned int *err = nullptr;
     long long *ovf = nullptr;
+    long long *ovf2 = nullptr;  // clique edges left for the HBM-row kernel
     long long ovf_cap = 0;
     int *gstore = nullptr;
     size_t gstore_bytes = 0;
@@ -1176,7 +1256,8 @@ void destroy_plan_device(PlanDevice *d) {
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
-                    (void *)d->err, (void *)d->ovf, (void *)d->gstore, (void *)d->grows, (void *)d->gsv})
+                    (void *)d->err, (void *)d->ovf, (void *)d->ovf2, (void *)d->gstore, (void *)d->grows,
+                    (void *)d->gsv})
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
@@ -1263,6 +1344,9 @@ struct Launch {
     gatk::CArgs ca;
     size_t clique_lds = 0;
     int clique_wave_bytes = 0;
+    gatk::CMidArgs cm;
+    dim3 mid_grid;
+    size_t mid_lds = 0;
     gatk::CBigArgs cb;
     dim3 big_grid;
     size_t big_lds = 0;
@@ -1323,9 +1407,12 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     const long long ovf_cap = clique ? g.m + 1 : (g.m + chunk - 1) / chunk + 1;
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
+        if (d->ovf2) HIPCHECK(hipFree(d->ovf2));
+        d->ovf = d->ovf2 = nullptr;
         HIPCHECK(hipMalloc(&d->ovf, sizeof(long long) * ovf_cap));
         d->ovf_cap = ovf_cap;
     }
+    if (clique && !d->ovf2) HIPCHECK(hipMalloc(&d->ovf2, sizeof(long long) * d->ovf_cap));
     const int wave_bytes = int((sizeof(gatk::WaveState) + size_t(pd.n_store) * cfg.slot_capacity * 4 + 15) / 16 * 16);
     const size_t lds = size_t(wave_bytes) * cfg.block_waves;
     if (lds > di.lds_per_block)
@@ -1422,8 +1509,18 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
             d->big_cap = cap;
             d->big_waves = waves;
         }
+        gatk::CMidArgs cm{};
+        cm.c = c;
+        cm.ovf2 = d->ovf2;
+        cm.ovf2_n = d->ovf_n + 1;
+        L->cm = cm;
+        L->mid_lds = (size_t)4 * 32 * gatk::CMIDW * (1 + gatk::CMIDW);
+        if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique mid tier");
+        L->mid_grid = dim3(std::max(64, (int)(di.lds_per_cu / L->mid_lds) * di.cus));
         gatk::CBigArgs cb{};
         cb.c = c;
+        cb.ovf2 = d->ovf2;
+        cb.ovf2_n = d->ovf_n + 1;
         cb.grows = d->grows;
         cb.gsv = d->gsv;
         cb.cap_s = d->big_cap;
@@ -1456,6 +1553,10 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
+        gatk::CMidArgs cm = L.cm;
+        cm.c.account = account ? 1 : 0;
+        cm.c.bytes = d->bytes + 1;
+        hipLaunchKernelGGL(gatk::clique_mid_kernel, L.mid_grid, dim3(64), L.mid_lds, s, cm);
         gatk::CBigArgs cb = L.cb;
         cb.c.account = account ? 1 : 0;
         cb.c.bytes = d->bytes + 1;

@@ -172,7 +172,7 @@ def test_bitmap_walg_matches_oracle(gat, orc, gpu):
         want_c, want_w = orc.clique_bitmap_walg(ro, ra, kmax)
         assert want_c == orc.unique_count(off, adj, clique(kmax))
         pats = [clique(k) for k in range(3, kmax + 1)]
-        for cap in (256, 32):  # 32: most edges go through the HBM-row kernel
+        for cap in (256, 32):  # 32: most edges go through the second (mid) tier
             got, st = gat.Plan(g, pats, "unique").count({"vertex_store": 2, "slot_capacity": cap}, account=True)
             if cap == 256:
                 assert st["overflow_tasks"] == 0
@@ -183,7 +183,8 @@ def test_bitmap_walg_matches_oracle(gat, orc, gpu):
 @pytest.mark.parametrize("gname", ["er200", "rmat8", "cl300"])
 def test_bitmap_clique_store(gat, orc, small_graphs, gname):
     """Bitmap rows (vertex_store 2) against the array store and the oracle;
-    slot_capacity 32 caps |S| at 32 so larger edges take the HBM-row kernel."""
+    slot_capacity 32 caps |S| at 32 in the main kernel so larger edges take
+    the second tier."""
     g = small_graphs[gname]
     off, adj = g.csr()
     ks = [3, 4, 5, 6]
