@@ -681,17 +681,24 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
 // Bitmap vertex store for clique plans (the reference declares
 // VertexStoreType::Bitmap in include/configs/vertex_set_config.hpp:3-6 but
 // never implements it). On the degree-oriented graph every k-clique is a
-// chain v0 > v1 > ... with v_{i+1} in N+(v_i). Per oriented edge (v0, v1)
-// the wave materialises S = N+(v0) ∩ N+(v1) (sorted, in LDS) and the
-// oriented adjacency of the subgraph induced on S as bit rows
-//   row_i = { j < i : S[j] in N+(S[i]) }.
-// Then #K3 = |S|, #K4 = sum_i popc(row_i), #K5 = sum_i sum_{j in row_i}
-// popc(row_i & row_j), #K6 one level deeper. Rows are built by one
-// flattened sweep over the lists N+(S[i]) (binary search in S, or the
-// reverse for skewed pairs), so the two deepest levels become LDS bit
-// operations instead of list intersections. Edges whose S outgrows the LDS
-// rows are queued for the array-store engine.
-constexpr int CMAXW = 8;  // row words: |S| <= 256
+// chain v0 > v1 > ... with v_{i+1} in N+(v_i). Per vertex v0 the wave stages
+// L = N+(v0) (sorted, d entries) and builds the oriented adjacency of the
+// subgraph induced on L as bit rows over positions in L:
+//   M_p = { q < p : L[q] in N+(L[p]) }.
+// Every clique with top vertex v0 is then a clique of these rows:
+//   #K3 += sum_p popc(M_p)
+//   #K4 += sum_p sum_{q in M_p} popc(M_p & M_q)
+//   #K5 += sum over q in M_p, r in M_p & M_q of popc(M_p & M_q & M_r)
+//   #K6 one level deeper.
+// Each neighbour pair inside N+(v0) is tested once per v0 (one flattened
+// sweep over the lists N+(L[p]), binary search in L or the reverse for
+// skewed pairs), so the deeper levels are pure bit operations. Tiers by d:
+// the main launch gives each vertex with d <= 256 one wave and 8-word rows in
+// LDS; larger vertices are shared by the 8 waves of a workgroup, with
+// 16-word rows in LDS (d <= 512) or rows in the workgroup's HBM scratch.
+constexpr int CMAXW = 8;   // main tier: d <= 256
+constexpr int CMIDW = 16;  // block tier in LDS: d <= 512
+constexpr int CBLOCKW = 8; // waves per workgroup in the block tiers
 
 struct CArgs {
     const long long *off;
@@ -702,75 +709,37 @@ struct CArgs {
     unsigned long long *counts;
     int pat_of_k[8];
     int kmax;
-    unsigned long long *queue;
-    long long *ovf;
+    unsigned long long *queue;   // [0] main vertex queue, [1] second tier, [2] HBM tier
+    long long *ovf;              // vertices for the second tier (count ovf_n[0])
+    long long *ovf2;             // vertices for the HBM tier (count ovf_n[1])
     unsigned long long *ovf_n;
     long long ovf_cap;
-    unsigned long long *bytes;
+    unsigned long long *bytes;   // W_alg accumulator of this launch
     unsigned int *err;
+    unsigned *grows;             // HBM tier: per-workgroup rows, hcap * ceil(hcap/32) words
+    int *goff;                   // HBM tier: per-workgroup pair offsets, hcap + 1 entries
+    int hcap;                    // HBM tier capacity (>= maximum out-degree)
     int account;
-    int cap_s;  // |S| limit for bit rows (<= 32 * CMAXW)
-    int cap0;   // N+(v0) staging entries
-    int chunk;
+    int cap;                     // main-tier limit on d (<= 32 * CMAXW)
+    int chunk;                   // vertices per main-queue grab
     int lin_thr;
     int gallop_ratio;
 };
 
-struct CliqueWave {
-    unsigned long long acc[8];
-    unsigned long long bytes;
-    int s_len;
-    int pad;
-};
-
-__device__ __forceinline__ int clique_wave_bytes(int cap0, int cap_s) {
-    const int w = (cap_s + 31) / 32;
-    return ((int)sizeof(CliqueWave) + 4 * cap0 + 4 * cap_s + 4 * cap_s * w + 15) / 16 * 16;
-}
-
-// S = A ∩ B for sorted lists; writes at most cap entries to out (LDS),
-// returns the full size. Streams the shorter list, searches the longer.
-template <class AP, class BP>
-__device__ __forceinline__ int clique_isect(AP A, int alen, BP B, int blen, int *out, int cap, int lin_thr,
-                                            int ln) {
-    int total = 0;
-    const bool sa = alen <= blen;
-    const int slen = sa ? alen : blen;
-    for (int base = 0; base < slen; base += W) {
-        const int i = base + ln;
-        bool hit = false;
-        int x = 0;
-        if (i < slen) {
-            x = sa ? A[i] : B[i];
-            hit = sa ? has(B, blen, x, lin_thr) : has(A, alen, x, lin_thr);
-        }
-        const unsigned long long mk = ballot(hit);
-        const int pos = total + mbcnt(mk);
-        if (hit && pos < cap) out[pos] = x;
-        total += __popcll(mk);
-    }
-    return total;
-}
-
-// Bit rows of the subgraph induced on S (LDS, s <= 32 * MW entries):
-// row_i = { j < i : S[j] in N+(S[i]) }, built by one flattened sweep over
-// the lists N+(S[i]); then #K4 += sum_i popc(row_i), #K5 += sum over set
-// bits j of row_i of popc(row_i & row_j), and #K6 one level deeper.
-template <int MW>
-__device__ void clique_rows_count(const CArgs &a, const int *sv, int s, unsigned *rows, int ln,
-                                  unsigned long long &acc4, unsigned long long &acc5, unsigned long long &acc6,
-                                  unsigned long long &bytes) {
-    // ---- bit rows of the subgraph induced on S ----
-    const int w = (s + 31) >> 5;
-    for (int k = ln; k < s * w; k += W) rows[k] = 0u;
-    wsync();
-    for (int cb = 0; cb < s; cb += W) {
+// Rows M_p of the list L (d entries) into rows (w words per row, already
+// zeroed), for the row blocks first, first + step, ... of 64 rows: lanes
+// sweep the (row, element) pairs of 64 rows at a time; each element of
+// N+(L[p]) is searched in L[0,p), or each L[q], q < p, in N+(L[p]) when that
+// list is gallop_ratio times longer than p.
+template <class LP, class RP>
+__device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first, int step, int ln,
+                         unsigned long long &bytes) {
+    for (int cb = first; cb < d; cb += step) {
         const int ci = cb + ln;
-        const bool valid = ci < s;
-        const int x = valid ? sv[ci] : 0;
+        const bool valid = ci < d;
+        const int x = valid ? L[ci] : 0;
         const long long xb = valid ? a.off[x] : 0;
         const int xl = valid ? a.split[x] : 0;  // N+(x): all below x
-        // stream N+(x) and search S[0,ci), or the reverse when skewed
         const bool sx = (long long)xl <= (long long)a.gallop_ratio * (long long)ci;
         const int mylen = valid ? (sx ? xl : ci) : 0;
         if (a.account && valid) bytes += 4ull * (xl + ci) + 16ull;
@@ -780,10 +749,10 @@ __device__ void clique_rows_count(const CArgs &a, const int *sv, int s, unsigned
             const int j = k0 + ln;
             int so = 0;
 #pragma unroll
-            for (int step = 32; step >= 1; step >>= 1)
-                if (bperm(myoff, so + step) <= j && so + step < W) so += step;
-            // every lane takes part in the cross-lane reads: a
-            // segment's owner lane may itself have no element here
+            for (int st = 32; st >= 1; st >>= 1)
+                if (bperm(myoff, so + st) <= j && so + st < W) so += st;
+            // every lane takes part in the cross-lane reads: a segment's
+            // owner lane may itself have no element here
             const int s_off = bperm(myoff, so);
             const int s_sx = bperm((int)sx, so);
             const long long s_xb = bperm64(xb, so);
@@ -794,53 +763,146 @@ __device__ void clique_rows_count(const CArgs &a, const int *sv, int s, unsigned
                 int pos = -1;
                 if (s_sx) {
                     const int y = a.adj[s_xb + idx];
-                    const int p = lbound(sv, s_ci, y);
-                    if (p < s_ci && sv[p] == y) pos = p;
+                    const int p = lbound(L, s_ci, y);
+                    if (p < s_ci && L[p] == y) pos = p;
                 } else {
-                    if (has(a.adj + s_xb, s_xl, sv[idx], a.lin_thr)) pos = idx;
+                    if (has(a.adj + s_xb, s_xl, L[idx], a.lin_thr)) pos = idx;
                 }
-                if (pos >= 0) atomicOr(&rows[s_ci * w + (pos >> 5)], 1u << (pos & 31));
+                if (pos >= 0) atomicOr(&rows[(long long)s_ci * w + (pos >> 5)], 1u << (pos & 31));
             }
         }
     }
-    wsync();
-    // ---- counts from the rows ----
-    for (int k = ln; k < s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
-    if (a.account && ln == 0) bytes += 4ull * s * w;
-    if (a.kmax >= 5) {
-        for (int ib = 0; ib < s; ib += W) {
-            const int ii = ib + ln;
-            if (ii >= s) continue;
-            unsigned ri[MW];
+}
+
+// Position of the j-th set bit (0-based) of x, j < popc(x).
+__device__ __forceinline__ int select_bit(unsigned x, int j) {
+    int pos = 0, c;
+    c = __popc(x & 0xffffu);
+    if (j >= c) { j -= c; x >>= 16; pos += 16; }
+    c = __popc(x & 0xffu);
+    if (j >= c) { j -= c; x >>= 8; pos += 8; }
+    c = __popc(x & 0xfu);
+    if (j >= c) { j -= c; x >>= 4; pos += 4; }
+    c = __popc(x & 0x3u);
+    if (j >= c) { j -= c; x >>= 2; pos += 2; }
+    if (j >= (int)(x & 1u)) pos += 1;
+    return pos;
+}
+
+// popc(M_p): row p has bits below p only.
+template <class RP>
+__device__ __forceinline__ int row_popc(RP rows, int p, int w) {
+    int c = 0;
+    for (int u = 0; u <= (p >> 5); u++) c += __popc(rows[(long long)p * w + u]);
+    return c;
+}
+
+// Counts at and below the pair level. The pairs (p, q), q in M_p, are
+// numbered through off[p] = sum of popc(M_p') over p' < p (off[d] = P), and
+// worker k takes pairs first, first + step, ..., so lanes share the work
+// evenly however uneven the rows are. For a pair: t = M_p & M_q gives #K4
+// += popc(t); each r in t adds popc(t & M_r) to #K5; each s in t & M_r adds
+// popc(t & M_r & M_s) to #K6. MW (words held in registers) must cover the
+// rows; MW == 0 reads every word from rows as needed (any width).
+template <int MW, class RP>
+__device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off, int P, int first, int step,
+                          unsigned long long *acc, unsigned long long &bytes) {
+    for (int k = first; k < P; k += step) {
+        int lo = 0, hi = d;  // off[lo] <= k < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= k) lo = mid; else hi = mid;
+        }
+        const int p = lo;
+        int j = k - off[p];
+        int q = 0;
+        for (int u = 0; u <= (p >> 5); u++) {
+            const unsigned x = rows[(long long)p * w + u];
+            const int c = __popc(x);
+            if (j < c) {
+                q = u * 32 + select_bit(x, j);
+                break;
+            }
+            j -= c;
+        }
+        const int wq = (q >> 5) + 1;
+        if (a.account) bytes += 8ull * w;
+        const long long pw = (long long)p * w, qw = (long long)q * w;
+        if constexpr (MW > 0) {
+            unsigned t[MW];
+            unsigned c4 = 0;
 #pragma unroll
-            for (int q = 0; q < MW; q++) ri[q] = q < w ? rows[ii * w + q] : 0u;
-            for (int q = 0; q < MW; q++) {
-                if (q >= w) break;
-                unsigned bits = ri[q];
-                while (bits) {
-                    const int jj = q * 32 + __ffs(bits) - 1;
-                    bits &= bits - 1;
-                    unsigned rm[MW];
-                    unsigned long long c5 = 0;
-#pragma unroll
-                    for (int t = 0; t < MW; t++) {
-                        rm[t] = t < w ? (ri[t] & rows[jj * w + t]) : 0u;
-                        c5 += (unsigned long long)__popc(rm[t]);
-                    }
-                    acc5 += c5;
+            for (int u = 0; u < MW; u++) {
+                t[u] = u < wq ? (rows[pw + u] & rows[qw + u]) : 0u;
+                c4 += __popc(t[u]);
+            }
+            acc[4] += c4;
+            if (a.kmax < 5) continue;
+            for (int rw = 0; rw < wq; rw++) {
+                unsigned br = t[rw];
+                while (br) {
+                    const int r = rw * 32 + __ffs(br) - 1;
+                    br &= br - 1;
+                    const int wr = (r >> 5) + 1;
+                    const long long rr = (long long)r * w;
                     if (a.account) bytes += 8ull * w;
-                    if (a.kmax >= 6) {
-                        for (int t = 0; t < MW; t++) {
-                            if (t >= w) break;
-                            unsigned b2 = rm[t];
-                            while (b2) {
-                                const int ll = t * 32 + __ffs(b2) - 1;
-                                b2 &= b2 - 1;
+                    if (a.kmax < 6) {
+                        unsigned c5 = 0;
 #pragma unroll
-                                for (int u = 0; u < MW; u++)
-                                    if (u < w) acc6 += (unsigned long long)__popc(rm[u] & rows[ll * w + u]);
-                                if (a.account) bytes += 8ull * w;
-                            }
+                        for (int u = 0; u < MW; u++)
+                            if (u < wr) c5 += __popc(t[u] & rows[rr + u]);
+                        acc[5] += c5;
+                        continue;
+                    }
+                    unsigned t2[MW];
+                    unsigned c5 = 0;
+#pragma unroll
+                    for (int u = 0; u < MW; u++) {
+                        t2[u] = u < wr ? (t[u] & rows[rr + u]) : 0u;
+                        c5 += __popc(t2[u]);
+                    }
+                    acc[5] += c5;
+                    unsigned c6 = 0;
+                    for (int sw = 0; sw < wr; sw++) {
+                        unsigned bs = t2[sw];
+                        while (bs) {
+                            const int sx = sw * 32 + __ffs(bs) - 1;
+                            bs &= bs - 1;
+                            const int wsx = (sx >> 5) + 1;
+                            const long long ss = (long long)sx * w;
+                            if (a.account) bytes += 8ull * w;
+#pragma unroll
+                            for (int u = 0; u < MW; u++)
+                                if (u < wsx) c6 += __popc(t2[u] & rows[ss + u]);
+                        }
+                    }
+                    acc[6] += c6;
+                }
+            }
+        } else {
+            for (int u = 0; u < wq; u++) acc[4] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u]);
+            if (a.kmax < 5) continue;
+            for (int rw = 0; rw < wq; rw++) {
+                unsigned br = rows[pw + rw] & rows[qw + rw];
+                while (br) {
+                    const int r = rw * 32 + __ffs(br) - 1;
+                    br &= br - 1;
+                    const int wr = (r >> 5) + 1;
+                    const long long rr = (long long)r * w;
+                    if (a.account) bytes += 8ull * w;
+                    for (int u = 0; u < wr; u++)
+                        acc[5] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
+                    if (a.kmax < 6) continue;
+                    for (int sw = 0; sw < wr; sw++) {
+                        unsigned bs = rows[pw + sw] & rows[qw + sw] & rows[rr + sw];
+                        while (bs) {
+                            const int sx = sw * 32 + __ffs(bs) - 1;
+                            bs &= bs - 1;
+                            const long long ss = (long long)sx * w;
+                            if (a.account) bytes += 8ull * w;
+                            for (int u = 0; u <= (sx >> 5); u++)
+                                acc[6] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u] & rows[rr + u] &
+                                                                     rows[ss + u]);
                         }
                     }
                 }
@@ -849,269 +911,154 @@ __device__ void clique_rows_count(const CArgs &a, const int *sv, int s, unsigned
     }
 }
 
+__device__ __forceinline__ void clique_flush(const CArgs &a, unsigned long long *acc, unsigned long long bytes,
+                                             int ln) {
+    for (int k = 3; k <= 6; k++) acc[k] = wave_sum64(acc[k]);
+    bytes = wave_sum64(bytes);
+    if (ln == 0) {
+        for (int k = 3; k <= 6; k++)
+            if (a.pat_of_k[k] >= 0 && acc[k]) atomicAdd(&a.counts[a.pat_of_k[k]], acc[k]);
+        if (a.account && bytes) atomicAdd(a.bytes, bytes);
+    }
+}
+
+// Main tier: one wave per vertex, vertices from the main queue, d <= a.cap
+// (rows in LDS, 8 words wide); larger vertices go to the block tiers.
 __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
     unsigned char *base = smem + (size_t)wid * wave_bytes;
-    CliqueWave *S = reinterpret_cast<CliqueWave *>(base);
-    int *n0 = reinterpret_cast<int *>(base + sizeof(CliqueWave));
-    int *sv = n0 + a.cap0;
-    unsigned *rows = reinterpret_cast<unsigned *>(sv + a.cap_s);
-    if (ln < 8) S->acc[ln] = 0;
-    if (ln == 0) S->bytes = 0;
-    wsync();
-    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;  // per lane
+    int *L = reinterpret_cast<int *>(base);
+    unsigned *rows = reinterpret_cast<unsigned *>(L + a.cap + 4);  // L: d entries, then d + 1 offsets
+    unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;  // per lane
     for (;;) {
         unsigned long long r = 0;
         if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
-        const long long e0 = uni64((long long)__shfl(r, 0));
-        if (e0 >= a.m) break;
-        const long long e1 = e0 + a.chunk < a.m ? e0 + a.chunk : a.m;
-        long long e = e0;
-        while (e < e1) {
-            int lo = 0, hi = a.n;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (a.off[mid] <= e) lo = mid; else hi = mid;
-            }
-            const int v0 = uni(lo);
-            const long long rb = a.off[v0], re = a.off[v0 + 1];
-            const long long se = e1 < re ? e1 : re;
-            const long long oe = rb + a.split[v0];  // out-edges end here
-            const long long hi_e = se < oe ? se : oe;
-            if (e < hi_e) {
-                const int need = (int)(hi_e - rb);  // N+(v0)[0, need) is all these edges look at
-                const bool staged = need <= a.cap0;
-                if (staged) {
-                    for (int k = ln; k < need; k += W) n0[k] = a.adj[rb + k];
-                    wsync();
+        const long long v_lo = uni64((long long)__shfl(r, 0));
+        if (v_lo >= a.n) break;
+        const long long v_hi = v_lo + a.chunk < a.n ? v_lo + a.chunk : a.n;
+        for (long long vv = v_lo; vv < v_hi; vv++) {
+            const int v0 = (int)vv;
+            const int d = uni(a.split[v0]);
+            if (d < 2) continue;
+            if (d > a.cap) {
+                if (ln == 0) {
+                    long long *list = d <= 32 * CMIDW ? a.ovf : a.ovf2;
+                    const unsigned long long idx = atomicAdd(&a.ovf_n[d <= 32 * CMIDW ? 0 : 1], 1ull);
+                    if ((long long)idx < a.ovf_cap) list[idx] = v0; else atomicOr(a.err, 4u);
                 }
-                for (long long ee = e; ee < hi_e; ee++) {
-                    const int i = (int)(ee - rb);
-                    const int v1 = uni(a.adj[ee]);
-                    const long long ab = a.off[v1];
-                    const int alen = a.split[v1];
-                    const int cap = a.kmax >= 4 ? a.cap_s : 0;
-                    int s = staged ? clique_isect(a.adj + ab, alen, n0, i, sv, cap, a.lin_thr, ln)
-                                   : clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, cap, a.lin_thr, ln);
-                    s = uni(s);
-                    if (a.account && ln == 0 && !(a.kmax >= 4 && s > a.cap_s))
-                        bytes += 4ull * (alen + i) + (a.kmax >= 4 ? 4ull * s : 0ull) + 32ull;
-                    if (a.kmax >= 4 && s > a.cap_s) {  // rows do not fit: the HBM-row kernel takes this edge
-                        if (ln == 0) {
-                            const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
-                            if ((long long)idx < a.ovf_cap) a.ovf[idx] = ee; else atomicOr(a.err, 4u);
-                        }
-                        continue;
-                    }
-                    if (ln == 0) acc3 += (unsigned long long)s;
-                    if (a.kmax < 4 || s < 2) continue;
-                    wsync();
-                    clique_rows_count<CMAXW>(a, sv, s, rows, ln, acc4, acc5, acc6, bytes);
-                    wsync();
-                }
+                continue;
             }
-            e = se;
-        }
-    }
-    acc3 = wave_sum64(acc3);
-    acc4 = wave_sum64(acc4);
-    acc5 = wave_sum64(acc5);
-    acc6 = wave_sum64(acc6);
-    bytes = wave_sum64(bytes);
-    if (ln == 0) {
-        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
-        for (int k = 3; k <= 6; k++)
-            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
-        if (a.account && bytes) atomicAdd(a.bytes, bytes);
-    }
-}
-
-// Second tier: edges whose S outgrew the main kernel's rows but has at most
-// 32 * CMIDW entries keep S and the rows in LDS (one wave per workgroup,
-// fewer waves per CU). Larger ones go on to the HBM-row kernel.
-constexpr int CMIDW = 16;  // |S| <= 512
-
-struct CMidArgs {
-    CArgs c;
-    long long *ovf2;             // edges for the HBM-row kernel
-    unsigned long long *ovf2_n;
-};
-
-__global__ void __launch_bounds__(64) clique_mid_kernel(CMidArgs m) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const CArgs &a = m.c;
-    const int ln = threadIdx.x % W;
-    constexpr int cap = 32 * CMIDW;
-    int *sv = reinterpret_cast<int *>(smem);
-    unsigned *rows = reinterpret_cast<unsigned *>(sv + cap);
-    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;
-    for (;;) {
-        unsigned long long r = 0;
-        if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
-        r = (unsigned long long)uni64((long long)__shfl(r, 0));
-        if (r >= *a.ovf_n || (long long)r >= a.ovf_cap) break;
-        const long long ee = a.ovf[r];
-        int lo = 0, hi = a.n;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.off[mid] <= ee) lo = mid; else hi = mid;
-        }
-        const int v0 = uni(lo);
-        const long long rb = a.off[v0];
-        const int i = (int)(ee - rb);
-        const int v1 = uni(a.adj[ee]);
-        const long long ab = a.off[v1];
-        const int alen = a.split[v1];
-        const int s = uni(clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, cap, a.lin_thr, ln));
-        if (s > cap) {
+            const long long rb = a.off[v0];
+            const int w = (d + 31) >> 5;
+            for (int k = ln; k < d; k += W) L[k] = a.adj[rb + k];
+            for (int k = ln; k < d * w; k += W) rows[k] = 0u;
+            if (a.account && ln == 0) bytes += 4ull * d + 16ull;
+            wsync();
+            vtx_rows(a, L, d, rows, w, 0, W, ln, bytes);
+            wsync();
+            // L is no longer needed: it holds the pair offsets from here on
+            int *off = L;
+            int run = 0;
+            for (int pb = 0; pb < d; pb += W) {
+                const int p = pb + ln;
+                const int c = p < d ? row_popc(rows, p, w) : 0;
+                if (a.account && p < d) bytes += 4ull * w;
+                int tot = 0;
+                const int x = excl_scan(c, &tot);
+                if (p < d) off[p] = run + x;
+                run += tot;
+            }
             if (ln == 0) {
-                const unsigned long long idx = atomicAdd(m.ovf2_n, 1ull);
-                if ((long long)idx < a.ovf_cap) m.ovf2[idx] = ee; else atomicOr(a.err, 16u);
+                off[d] = run;
+                acc[3] += (unsigned long long)run;
             }
-            continue;
+            wsync();
+            if (a.kmax >= 4) vtx_pairs<CMAXW>(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
+            wsync();
         }
-        if (a.account && ln == 0) bytes += 4ull * (alen + i) + 4ull * s + 32ull;
-        if (ln == 0) acc3 += (unsigned long long)s;
-        if (s < 2) continue;
-        wsync();
-        clique_rows_count<CMIDW>(a, sv, s, rows, ln, acc4, acc5, acc6, bytes);
-        wsync();
     }
-    acc3 = wave_sum64(acc3);
-    acc4 = wave_sum64(acc4);
-    acc5 = wave_sum64(acc5);
-    acc6 = wave_sum64(acc6);
-    bytes = wave_sum64(bytes);
-    if (ln == 0) {
-        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
-        for (int k = 3; k <= 6; k++)
-            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
-        if (a.account && bytes) atomicAdd(a.bytes, bytes);
-    }
+    clique_flush(a, acc, bytes, ln);
 }
 
-// Edges whose S outgrows both LDS tiers: one edge per work item, S in LDS
-// (or HBM when the maximum out-degree is too large for LDS), bit rows in a
-// per-wave HBM/L2 scratch area of capacity cap_s x ceil(cap_s/32) words.
-struct CBigArgs {
-    CArgs c;
-    unsigned *grows;  // per-wave rows
-    int *gsv;         // per-wave S when not in LDS
-    int cap_s;        // >= maximum out-degree
-    int sv_lds;       // S staged in LDS
-    const long long *ovf2;  // edges left by the LDS tiers
-    const unsigned long long *ovf2_n;
-};
-
-__global__ void __launch_bounds__(64) clique_big_kernel(CBigArgs b) {
+// Block tiers: the NW waves of a workgroup share one vertex. LDS: rows of
+// vertices with d <= 32 * CMIDW, 16 words wide; HBM: any d up to hcap, rows
+// in the workgroup's scratch. The waves split the rows between them for the
+// build and for the count.
+template <bool LDS, int NW>
+__global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const CArgs &a = b.c;
-    const int ln = threadIdx.x % W;
-    const int wave = blockIdx.x;
-    const int ws = (b.cap_s + 31) / 32;
-    unsigned *rows = b.grows + (long long)wave * b.cap_s * ws;
-    int *sv = b.sv_lds ? reinterpret_cast<int *>(smem) : b.gsv + (long long)wave * b.cap_s;
-    unsigned long long acc3 = 0, acc4 = 0, acc5 = 0, acc6 = 0, bytes = 0;
+    const int wid = threadIdx.x / W, ln = threadIdx.x % W;
+    constexpr int lcap = 32 * CMIDW;
+    // LDS tier: the list (then the pair offsets), then the rows
+    int *Ls = reinterpret_cast<int *>(smem);
+    unsigned *rows_l = reinterpret_cast<unsigned *>(Ls + (LDS ? lcap + 4 : 0));
+    __shared__ int vsh;
+    __shared__ int wtot[NW];
+    const int hws = (a.hcap + 31) / 32;
+    unsigned *rows_g = a.grows + (long long)blockIdx.x * a.hcap * hws;
+    int *off_g = a.goff + (long long)blockIdx.x * (a.hcap + 1);
+    unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;
     for (;;) {
-        unsigned long long r = 0;
-        if (ln == 0) r = atomicAdd(&a.queue[2], 1ull);
-        r = (unsigned long long)uni64((long long)__shfl(r, 0));
-        if (r >= *b.ovf2_n || (long long)r >= a.ovf_cap) break;
-        const long long ee = b.ovf2[r];
-        int lo = 0, hi = a.n;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.off[mid] <= ee) lo = mid; else hi = mid;
+        if (threadIdx.x == 0) {
+            const unsigned long long r = atomicAdd(&a.queue[LDS ? 1 : 2], 1ull);
+            const unsigned long long cnt = a.ovf_n[LDS ? 0 : 1];
+            vsh = (r < cnt && (long long)r < a.ovf_cap) ? (int)(LDS ? a.ovf[r] : a.ovf2[r]) : -1;
         }
-        const int v0 = uni(lo);
+        __syncthreads();
+        const int v0 = vsh;
+        if (v0 < 0) break;
+        const int d = a.split[v0];
         const long long rb = a.off[v0];
-        const int i = (int)(ee - rb);
-        const int v1 = uni(a.adj[ee]);
-        const long long ab = a.off[v1];
-        const int alen = a.split[v1];
-        int s = uni(clique_isect(a.adj + ab, alen, a.adj + rb, i, sv, b.cap_s, a.lin_thr, ln));
-        if (s > b.cap_s) {  // cannot happen: S lies in N+(v1)
-            if (ln == 0) atomicOr(a.err, 8u);
-            continue;
-        }
-        if (a.account && ln == 0) bytes += 4ull * (alen + i) + 4ull * s + 32ull;
-        if (ln == 0) acc3 += (unsigned long long)s;
-        if (s < 2) continue;
-        const int w = (s + 31) >> 5;
-        for (long long k = ln; k < (long long)s * w; k += W) rows[k] = 0u;
-        __threadfence();
-        wsync();
-        // rows: lanes take elements of S; each streams N+(x) against S[0,ci)
-        // (or the reverse when skewed) -- here the lists are long enough for
-        // a lane-per-row sweep
-        for (int ci = ln; ci < s; ci += W) {
-            const int x = sv[ci];
-            const long long xb = a.off[x];
-            const int xl = a.split[x];
-            if (a.account) bytes += 4ull * (xl + ci) + 16ull;
-            if ((long long)xl <= (long long)a.gallop_ratio * ci) {
-                for (int k = 0; k < xl; k++) {
-                    const int y = a.adj[xb + k];
-                    const int p = lbound(sv, ci, y);
-                    if (p < ci && sv[p] == y) rows[(long long)ci * w + (p >> 5)] |= 1u << (p & 31);
+        const int w = (d + 31) >> 5;
+        if (d > (LDS ? lcap : a.hcap)) {  // cannot happen: lists are split by d
+            if (threadIdx.x == 0) atomicOr(a.err, 8u);
+        } else {
+            if (LDS)
+                for (int k = threadIdx.x; k < d; k += 64 * NW) Ls[k] = a.adj[rb + k];
+            unsigned *rows = LDS ? rows_l : rows_g;
+            for (long long k = threadIdx.x; k < (long long)d * w; k += 64 * NW) rows[k] = 0u;
+            if (a.account && threadIdx.x == 0) bytes += 4ull * d + 16ull;
+            if (!LDS) __threadfence();
+            __syncthreads();
+            const int *L = LDS ? Ls : a.adj + rb;
+            vtx_rows(a, L, d, rows, w, wid * W, W * NW, ln, bytes);
+            if (!LDS) __threadfence();
+            __syncthreads();
+            // pair offsets: block-wide exclusive prefix of popc(M_p)
+            int *off = LDS ? Ls : off_g;
+            int run = 0;
+            for (int pb = 0; pb < d; pb += 64 * NW) {
+                const int p = pb + (int)threadIdx.x;
+                const int c = p < d ? row_popc((const unsigned *)rows, p, w) : 0;
+                if (a.account && p < d) bytes += 4ull * w;
+                int tot = 0;
+                const int x = excl_scan(c, &tot);
+                if (ln == 0) wtot[wid] = tot;
+                __syncthreads();
+                int pre = 0, all = 0;
+                for (int k = 0; k < NW; k++) {
+                    const int t = wtot[k];
+                    if (k < wid) pre += t;
+                    all += t;
                 }
-            } else {
-                for (int k = 0; k < ci; k++)
-                    if (has(a.adj + xb, xl, sv[k], a.lin_thr)) rows[(long long)ci * w + (k >> 5)] |= 1u << (k & 31);
+                if (p < d) off[p] = run + pre + x;
+                run += all;
+                __syncthreads();
             }
-        }
-        __threadfence();
-        wsync();
-        for (long long k = ln; k < (long long)s * w; k += W) acc4 += (unsigned long long)__popc(rows[k]);
-        if (a.account && ln == 0) bytes += 4ull * s * w;
-        if (a.kmax >= 5) {
-            // lanes take the words of row i; each set bit j adds popc(row_i & row_j)
-            for (int ii = 0; ii < s; ii++) {
-                const unsigned *ri = rows + (long long)ii * w;
-                for (int q = ln; q < w; q += W) {
-                    unsigned bits = ri[q];
-                    while (bits) {
-                        const int jj = q * 32 + __ffs(bits) - 1;
-                        bits &= bits - 1;
-                        const unsigned *rj = rows + (long long)jj * w;
-                        const int wj = (jj >> 5) + 1;  // row j has bits below j only
-                        unsigned long long c5 = 0;
-                        for (int t = 0; t < wj; t++) c5 += (unsigned long long)__popc(ri[t] & rj[t]);
-                        acc5 += c5;
-                        if (a.account) bytes += 8ull * w;
-                        if (a.kmax >= 6) {
-                            for (int t = 0; t < wj; t++) {
-                                unsigned b2 = ri[t] & rj[t];
-                                while (b2) {
-                                    const int ll = t * 32 + __ffs(b2) - 1;
-                                    b2 &= b2 - 1;
-                                    const unsigned *rl = rows + (long long)ll * w;
-                                    const int wl = (ll >> 5) + 1;
-                                    for (int u = 0; u < wl; u++)
-                                        acc6 += (unsigned long long)__popc(ri[u] & rj[u] & rl[u]);
-                                    if (a.account) bytes += 8ull * w;
-                                }
-                            }
-                        }
-                    }
-                }
+            if (threadIdx.x == 0) {
+                off[d] = run;
+                acc[3] += (unsigned long long)run;
             }
+            if (!LDS) __threadfence();
+            __syncthreads();
+            if (a.kmax >= 4)
+                vtx_pairs<LDS ? CMIDW : 0>(a, (const unsigned *)rows, d, w, off, run, (int)threadIdx.x, 64 * NW, acc,
+                                           bytes);
         }
-        __threadfence();
-        wsync();
+        __syncthreads();
     }
-    acc3 = wave_sum64(acc3);
-    acc4 = wave_sum64(acc4);
-    acc5 = wave_sum64(acc5);
-    acc6 = wave_sum64(acc6);
-    bytes = wave_sum64(bytes);
-    if (ln == 0) {
-        const unsigned long long v[7] = {0, 0, 0, acc3, acc4, acc5, acc6};
-        for (int k = 3; k <= 6; k++)
-            if (a.pat_of_k[k] >= 0 && v[k]) atomicAdd(&a.counts[a.pat_of_k[k]], v[k]);
-        if (a.account && bytes) atomicAdd(a.bytes, bytes);
-    }
+    clique_flush(a, acc, bytes, ln);
 }
 
 __global__ void fill_u64(unsigned long long *p, long long n, unsigned long long v) {
@@ -1239,9 +1186,9 @@ struct PlanDevice {
     size_t gstore_bytes = 0;
     int ovf_grid = 0;
     hipEvent_t ev[4] = {};
-    // heavy-edge bitmap scratch
+    // clique HBM-tier rows and pair offsets
     unsigned *grows = nullptr;
-    int *gsv = nullptr;
+    int *goff = nullptr;
     int big_waves = 0, big_cap = 0;
     std::shared_ptr<DeviceGraph> dg;
     // captured graph
@@ -1257,7 +1204,7 @@ void destroy_plan_device(PlanDevice *d) {
         if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
                     (void *)d->err, (void *)d->ovf, (void *)d->ovf2, (void *)d->gstore, (void *)d->grows,
-                    (void *)d->gsv})
+                    (void *)d->goff})
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
@@ -1344,12 +1291,9 @@ struct Launch {
     gatk::CArgs ca;
     size_t clique_lds = 0;
     int clique_wave_bytes = 0;
-    gatk::CMidArgs cm;
     dim3 mid_grid;
     size_t mid_lds = 0;
-    gatk::CBigArgs cb;
     dim3 big_grid;
-    size_t big_lds = 0;
 };
 
 static void fill_defaults(gat_kernel_config *c) {
@@ -1470,23 +1414,17 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         for (int k = 0; k < 8; k++) c.pat_of_k[k] = p->cp.clique_pattern[k];
         c.kmax = 31 - __builtin_clz(p->cp.clique_ks);
         c.queue = a.queue;
-        c.ovf = a.ovf;
+        c.ovf = d->ovf;
+        c.ovf2 = d->ovf2;
         c.ovf_n = a.ovf_n;
-        c.ovf_cap = a.ovf_cap;
+        c.ovf_cap = d->ovf_cap;
         c.bytes = a.bytes;
         c.err = a.err;
-        c.cap_s = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
-        c.cap0 = std::max(256, std::min(4096, cfg.slot_capacity));
+        c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
         c.chunk = cfg.queue_chunk;
         c.lin_thr = cfg.search_linear_threshold;
         c.gallop_ratio = cfg.gallop_ratio;
-        const int w = (c.cap_s + 31) / 32;
-        L->clique_wave_bytes = ((int)sizeof(gatk::CliqueWave) + 4 * c.cap0 + 4 * c.cap_s + 4 * c.cap_s * w + 15) / 16 * 16;
-        L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
-        if (L->clique_lds > di.lds_per_block)
-            GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
-        L->ca = c;
-        // heavy edges: rows in HBM sized by the maximum out-degree
+        // HBM tier: rows sized by the maximum out-degree
         if (g.max_out < 0) {
             int32_t mo = 0;
             for (int32_t v = 0; v < g.n; v++) {
@@ -1495,39 +1433,32 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
             }
             g.max_out = mo;
         }
-        const int cap = std::max(32, g.max_out);
-        const int ws = (cap + 31) / 32;
+        const int hcap = std::max(32, g.max_out);
+        const int ws = (hcap + 31) / 32;
         int waves = std::max(64, 8 * di.cus);
-        while (waves > 64 && (double)waves * cap * ws * 4 > 16e9) waves /= 2;
-        if (d->big_cap < cap || d->big_waves < waves) {
+        while (waves > 64 && (double)waves * hcap * ws * 4 > 16e9) waves /= 2;
+        if (d->big_cap < hcap || d->big_waves < waves) {
             if (d->grows) HIPCHECK(hipFree(d->grows));
-            if (d->gsv) HIPCHECK(hipFree(d->gsv));
+            if (d->goff) HIPCHECK(hipFree(d->goff));
             d->grows = nullptr;
-            d->gsv = nullptr;
-            HIPCHECK(hipMalloc(&d->grows, (size_t)waves * cap * ws * 4));
-            HIPCHECK(hipMalloc(&d->gsv, (size_t)waves * cap * 4));
-            d->big_cap = cap;
+            d->goff = nullptr;
+            HIPCHECK(hipMalloc(&d->grows, (size_t)waves * hcap * ws * 4));
+            HIPCHECK(hipMalloc(&d->goff, (size_t)waves * (hcap + 1) * 4));
+            d->big_cap = hcap;
             d->big_waves = waves;
         }
-        gatk::CMidArgs cm{};
-        cm.c = c;
-        cm.ovf2 = d->ovf2;
-        cm.ovf2_n = d->ovf_n + 1;
-        L->cm = cm;
-        L->mid_lds = (size_t)4 * 32 * gatk::CMIDW * (1 + gatk::CMIDW);
-        if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique mid tier");
-        L->mid_grid = dim3(std::max(64, (int)(di.lds_per_cu / L->mid_lds) * di.cus));
-        gatk::CBigArgs cb{};
-        cb.c = c;
-        cb.ovf2 = d->ovf2;
-        cb.ovf2_n = d->ovf_n + 1;
-        cb.grows = d->grows;
-        cb.gsv = d->gsv;
-        cb.cap_s = d->big_cap;
-        cb.sv_lds = (size_t)cap * 4 <= std::min<size_t>(48 * 1024, di.lds_per_block) ? 1 : 0;
-        L->cb = cb;
-        L->big_grid = dim3(d->big_waves);
-        L->big_lds = cb.sv_lds ? (size_t)d->big_cap * 4 : 0;
+        c.grows = d->grows;
+        c.goff = d->goff;
+        c.hcap = d->big_cap;
+        L->ca = c;
+        L->clique_wave_bytes = (4 * (c.cap + 4) + 4 * c.cap * (c.cap / 32) + 15) / 16 * 16;
+        L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
+        if (L->clique_lds > di.lds_per_block)
+            GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
+        L->mid_lds = (size_t)4 * (32 * gatk::CMIDW + 4) + (size_t)4 * 32 * gatk::CMIDW * gatk::CMIDW;
+        if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique block tier");
+        L->mid_grid = dim3(std::max(1, (int)(di.lds_per_cu / L->mid_lds)) * di.cus);
+        L->big_grid = dim3(std::min(d->big_waves, 2 * di.cus));  // one scratch area per workgroup
     }
     return GAT_OK;
 }
@@ -1544,23 +1475,20 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     a.account = ao.account = account ? 1 : 0;
     ao.bytes = d->bytes + 1;  // second launch: its own W_alg counter
     if (events) HIPCHECK(hipEventRecord(d->ev[0], s));
+    gatk::CArgs c = L.ca;
+    c.account = account ? 1 : 0;
     if (L.clique) {
-        gatk::CArgs c = L.ca;
-        c.account = account ? 1 : 0;
         hipLaunchKernelGGL(gatk::clique_kernel, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
     } else {
         hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, a, L.wave_bytes);
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
-        gatk::CMidArgs cm = L.cm;
-        cm.c.account = account ? 1 : 0;
-        cm.c.bytes = d->bytes + 1;
-        hipLaunchKernelGGL(gatk::clique_mid_kernel, L.mid_grid, dim3(64), L.mid_lds, s, cm);
-        gatk::CBigArgs cb = L.cb;
-        cb.c.account = account ? 1 : 0;
-        cb.c.bytes = d->bytes + 1;
-        hipLaunchKernelGGL(gatk::clique_big_kernel, L.big_grid, dim3(64), L.big_lds, s, cb);
+        c.bytes = d->bytes + 1;  // later tiers: their own W_alg counter
+        hipLaunchKernelGGL((gatk::clique_block_kernel<true, gatk::CBLOCKW>), L.mid_grid, dim3(64 * gatk::CBLOCKW),
+                           L.mid_lds, s, c);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<false, gatk::CBLOCKW>), L.big_grid, dim3(64 * gatk::CBLOCKW), 0,
+                           s, c);
     } else {
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
     }
@@ -1572,10 +1500,10 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
 static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf = 0, bytes[2] = {0, 0};
+    unsigned long long raw[GATP_MAX_TERMS], ovf[2] = {0, 0}, bytes[2] = {0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
-    HIPCHECK(hipMemcpyAsync(&ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
+    HIPCHECK(hipMemcpyAsync(ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(bytes, d->bytes, sizeof(bytes), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(&err, d->err, sizeof(err), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipStreamSynchronize(d->stream));
@@ -1596,7 +1524,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->seconds = ms / 1000.0;
             st->main_kernel_ms = ms_main;
         }
-        st->overflow_tasks = ovf;
+        st->overflow_tasks = ovf[0] + ovf[1];
         st->alg_bytes = bytes[0] + bytes[1];
         st->alg_bytes_main = bytes[0];
         st->tasks = (uint64_t)p->graph->m;

@@ -135,7 +135,9 @@ typedef struct gat_kernel_config {
 typedef struct gat_count_stats {
     double seconds;           /* device time of the count (hipEvents)        */
     double main_kernel_ms;    /* average duration of the main engine launch  */
-    uint64_t overflow_tasks;  /* tasks re-run with HBM-resident prefix sets  */
+    uint64_t overflow_tasks;  /* array store: tasks re-run with HBM-resident
+                                 prefix sets; bitmap store: vertices passed to
+                                 the workgroup tiers                          */
     uint64_t alg_bytes;       /* W_alg of the whole count (GAT_FLAG_ACCOUNT)  */
     uint64_t alg_bytes_main;  /* W_alg of the main launch alone              */
     uint64_t tasks;           /* directed edges in the task space            */

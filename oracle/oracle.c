@@ -1040,61 +1040,56 @@ void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off
     free(nw);
 }
 
-/* Algorithmic bytes of the bitmap clique path (cliques K3..kmax, kmax <= 6)
- * on a descending-degree graph, assuming every S fits the bit rows:
- * per oriented edge 4(|N+(v1)| + i) + 4|S| (rows needed) + 32; per row
- * 4(|N+(x)| + ci) + 16; 4|S|w for the K4 popcounts; 8w per (i,j) pair for
- * K5 and per (i,j,l) triple for K6 (w = ceil(|S|/32)). Returns #K_kmax. */
+/* Algorithmic bytes of the bitmap clique store (cliques K3..kmax, kmax <= 6)
+ * on a descending-degree graph. Per vertex v0 with d = |N+(v0)| >= 2
+ * (L = N+(v0), w = ceil(d/32)): 4d + 16 to read L; per position p of L,
+ * 4(|N+(L[p])| + p) + 16 to build row M_p = {q < p : L[q] in N+(L[p])} and
+ * 4w to count it; 8w per pair (p,q), q in M_p (when kmax >= 4), per triple
+ * (kmax >= 5) and per quadruple (kmax >= 6) of mutually adjacent positions.
+ * Every clique with top vertex v0 is v0 plus a clique of these rows, so
+ * #K3 = #pairs, #K4 = #triples, ... Returns #K_kmax. */
 uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *adj, int kmax, uint64_t *walg) {
     int32_t v0, maxd = 0, v;
     uint64_t bytes = 0, cnt[7] = {0};
-    int32_t *S;
-    unsigned char *M; /* dense adjacency among S: M[i*s+j] */
+    unsigned char *M; /* M[p*d+q] = 1 iff q < p and L[q] in N+(L[p]) */
     for (v = 0; v < n; v++)
         if (off[v + 1] - off[v] > maxd) maxd = (int32_t)(off[v + 1] - off[v]);
-    S = (int32_t *)malloc(sizeof(int32_t) * (maxd + 1));
     M = (unsigned char *)malloc((size_t)(maxd + 1) * (maxd + 1));
     for (v0 = 0; v0 < n; v0++) {
-        const int32_t *n0 = adj + off[v0];
-        int64_t a0 = lower_bound_i32(n0, off[v0 + 1] - off[v0], v0), i;
-        for (i = 0; i < a0; i++) {
-            int32_t v1 = n0[i];
-            const int32_t *n1 = adj + off[v1];
-            int64_t a = lower_bound_i32(n1, off[v1 + 1] - off[v1], v1), j, s = 0, w, ci, cj, cl;
-            for (j = 0; j < a; j++)
-                if (in_sorted(n0, i, n1[j])) S[s++] = n1[j];
-            bytes += 4ull * (uint64_t)(a + i) + (kmax >= 4 ? 4ull * (uint64_t)s : 0ull) + 32ull;
-            cnt[3] += (uint64_t)s;
-            if (kmax < 4 || s < 2) continue;
-            w = (s + 31) / 32;
-            for (ci = 0; ci < s; ci++) {
-                int32_t x = S[ci];
-                int64_t xl = lower_bound_i32(adj + off[x], off[x + 1] - off[x], x);
-                bytes += 4ull * (uint64_t)(xl + ci) + 16ull;
-                for (cj = 0; cj < s; cj++) M[ci * s + cj] = cj < ci && in_sorted(adj + off[x], xl, S[cj]);
-            }
-            bytes += 4ull * (uint64_t)(s * w);
-            for (ci = 0; ci < s; ci++)
-                for (cj = 0; cj < ci; cj++) {
-                    if (!M[ci * s + cj]) continue;
+        const int32_t *L = adj + off[v0];
+        const int64_t d = lower_bound_i32(L, off[v0 + 1] - off[v0], v0);
+        int64_t w, p, q, r, t, u;
+        if (d < 2) continue;
+        w = (d + 31) / 32;
+        bytes += 4ull * (uint64_t)d + 16ull;
+        for (p = 0; p < d; p++) {
+            const int32_t x = L[p];
+            const int64_t xl = lower_bound_i32(adj + off[x], off[x + 1] - off[x], x);
+            bytes += 4ull * (uint64_t)(xl + p) + 16ull + 4ull * (uint64_t)w;
+            for (q = 0; q < d; q++) M[p * d + q] = q < p && in_sorted(adj + off[x], xl, L[q]);
+        }
+        for (p = 0; p < d; p++)
+            for (q = 0; q < p; q++) {
+                if (!M[p * d + q]) continue;
+                cnt[3]++;
+                if (kmax < 4) continue;
+                bytes += 8ull * (uint64_t)w;
+                for (r = 0; r < q; r++) {
+                    if (!(M[p * d + r] && M[q * d + r])) continue;
                     cnt[4]++;
                     if (kmax < 5) continue;
                     bytes += 8ull * (uint64_t)w;
-                    for (cl = 0; cl < cj; cl++) {
-                        if (!(M[ci * s + cl] && M[cj * s + cl])) continue;
+                    for (t = 0; t < r; t++) {
+                        if (!(M[p * d + t] && M[q * d + t] && M[r * d + t])) continue;
                         cnt[5]++;
                         if (kmax < 6) continue;
                         bytes += 8ull * (uint64_t)w;
-                        {
-                            int64_t cm;
-                            for (cm = 0; cm < cl; cm++)
-                                if (M[ci * s + cm] && M[cj * s + cm] && M[cl * s + cm]) cnt[6]++;
-                        }
+                        for (u = 0; u < t; u++)
+                            if (M[p * d + u] && M[q * d + u] && M[r * d + u] && M[t * d + u]) cnt[6]++;
                     }
                 }
-        }
+            }
     }
-    free(S);
     free(M);
     *walg = bytes;
     return cnt[kmax];

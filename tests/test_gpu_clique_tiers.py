@@ -1,8 +1,8 @@
 """The three tiers of the bitmap clique store (GPU): the main kernel keeps
-|S| <= slot_capacity (at most 256) in LDS, the second tier |S| <= 512 in
-LDS, and the HBM-row kernel takes the rest. Complete graphs pin every tier
-by closed forms: on K_m every oriented edge (v0, v1), v1 < v0, has
-S = {u < v1}, so |S| takes every value up to m - 2."""
+the rows of vertices with d = |N+(v0)| <= slot_capacity (at most 256) in
+LDS, the second tier d <= 512 in LDS, and the HBM-row kernel takes the rest.
+Complete graphs pin every tier by closed forms: on K_m vertex v has
+N+(v) = {u < v}, so d takes every value up to m - 1."""
 import math
 
 import pytest
@@ -20,13 +20,13 @@ def complete(gat, m):
 
 @pytest.mark.parametrize("cap", [256, 32, 64])
 def test_complete_graph_all_tiers(gat, gpu, cap):
-    m = 530  # 136 edges with |S| > 512 reach the HBM-row kernel
+    m = 530  # vertices 513..529 (d > 512) reach the HBM-row kernel
     g = complete(gat, m)
     ks = [3, 4, 5]
     got, st = gat.Plan(g, [clique(k) for k in ks], "unique").count({"vertex_store": 2, "slot_capacity": cap})
     assert got == [math.comb(m, k) for k in ks]
-    # edges with |S| > cap leave the main kernel: v1 in (cap, m - 2]
-    assert st["overflow_tasks"] == sum(m - 1 - v1 for v1 in range(cap + 1, m - 1))
+    # vertices with d > cap leave the main kernel: v in (cap, m - 1]
+    assert st["overflow_tasks"] == m - 1 - cap
     lab, _ = gat.Plan(g, [clique(4)], "labeled").count({"vertex_store": 2, "slot_capacity": cap})
     assert lab == [24 * math.comb(m, 4)]
 

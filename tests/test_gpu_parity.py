@@ -183,8 +183,8 @@ def test_bitmap_walg_matches_oracle(gat, orc, gpu):
 @pytest.mark.parametrize("gname", ["er200", "rmat8", "cl300"])
 def test_bitmap_clique_store(gat, orc, small_graphs, gname):
     """Bitmap rows (vertex_store 2) against the array store and the oracle;
-    slot_capacity 32 caps |S| at 32 in the main kernel so larger edges take
-    the second tier."""
+    slot_capacity 32 caps d = |N+(v0)| at 32 in the main kernel so larger
+    vertices take the second tier."""
     g = small_graphs[gname]
     off, adj = g.csr()
     ks = [3, 4, 5, 6]
@@ -201,9 +201,9 @@ def test_bitmap_clique_store(gat, orc, small_graphs, gname):
 
 
 def test_bitmap_overflow_and_labeled(gat, orc, gpu):
-    # dense graph: many S sets exceed a 32-entry cap
-    # (max |S| is 44). K4 from the oracle; K5 from the array store, which
-    # the tests above pin to the oracle (the oracle's K5 here takes ~30 s).
+    # dense graph: many N+(v0) exceed a 32-entry cap. K4 from the oracle; K5
+    # from the array store, which the tests above pin to the oracle (the
+    # oracle's K5 here takes ~30 s).
     g = gat.Graph.erdos_renyi(60, 0.9, 4)
     off, adj = g.csr()
     arr, _ = gat.Plan(g, [clique(4), clique(5)], "unique").count({"vertex_store": 1})

@@ -249,3 +249,35 @@ def test_relabel_and_clique_walg(orc):
                     tot += 4 * (len(A) + len(B)) + 4 * len(out) + 32
                     rec(out, 2, [v0, v1])
         assert w == tot
+
+
+def test_clique_bitmap_walg_definition(orc):
+    """Counts of the bitmap-store restatement equal the brute-force unique
+    counts, and its W_alg equals the definition recomputed here: per v0 with
+    d = |N+(v0)| >= 2, 4d + 16; per position p, 4(|N+(L[p])| + p) + 16 + 4w;
+    8w per pair, triple and quadruple of mutually adjacent positions that
+    the deeper levels visit (w = ceil(d/32))."""
+    off, adj = random_graph(45, 0.3, 7)
+    ro, ra = orc.relabel(off, adj)
+    n = len(ro) - 1
+    up = [[x for x in ra[ro[v]:ro[v + 1]] if x < v] for v in range(n)]
+    for k in (3, 4, 5, 6):
+        c, w = orc.clique_bitmap_walg(ro, ra, k)
+        assert c == orc.unique_count(off, adj, clique(k))
+        tot = 0
+        for v0 in range(n):
+            L = up[v0]
+            d = len(L)
+            if d < 2:
+                continue
+            wd = (d + 31) // 32
+            tot += 4 * d + 16 + sum(4 * (len(up[x]) + p) + 16 + 4 * wd for p, x in enumerate(L))
+            adjL = {(p, q) for p in range(d) for q in range(p) if L[q] in set(up[L[p]])}
+            # cliques of size j among positions, visited at depth j (j = 2, 3, 4)
+            for size in (2, 3, 4):
+                if k < size + 2:
+                    break
+                for combo in itertools.combinations(range(d), size):
+                    if all((b, a) in adjL for a, b in itertools.combinations(combo, 2)):
+                        tot += 8 * wd
+        assert w == tot, (k, w, tot)
