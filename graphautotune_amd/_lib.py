@@ -100,9 +100,11 @@ class CountStats(C.Structure):
     _fields_ = [
         ("seconds", C.c_double),
         ("main_kernel_ms", C.c_double),
+        ("second_kernel_ms", C.c_double),
         ("overflow_tasks", C.c_uint64),
         ("alg_bytes", C.c_uint64),
         ("alg_bytes_main", C.c_uint64),
+        ("alg_bytes_second", C.c_uint64),
         ("tasks", C.c_uint64),
         ("grid_blocks", C.c_int32),
         ("block_threads", C.c_int32),

@@ -697,8 +697,9 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
 // LDS; larger vertices are shared by the 8 waves of a workgroup, with
 // 16-word rows in LDS (d <= 512) or rows in the workgroup's HBM scratch.
 constexpr int CMAXW = 8;   // main tier: d <= 256
-constexpr int CMIDW = 16;  // block tier in LDS: d <= 512
-constexpr int CBLOCKW = 8; // waves per workgroup in the block tiers
+constexpr int CMIDW = 16;  // workgroup tier 0: d <= 512, rows in LDS
+constexpr int CBIGW = 32;  // workgroup tier 1: d <= 1024, rows in LDS when it fits, else HBM
+constexpr int CBLOCKW = 8; // waves per workgroup in the workgroup tiers
 
 struct CArgs {
     const long long *off;
@@ -709,9 +710,8 @@ struct CArgs {
     unsigned long long *counts;
     int pat_of_k[8];
     int kmax;
-    unsigned long long *queue;   // [0] main vertex queue, [1] second tier, [2] HBM tier
-    long long *ovf;              // vertices for the second tier (count ovf_n[0])
-    long long *ovf2;             // vertices for the HBM tier (count ovf_n[1])
+    unsigned long long *queue;   // [0] main vertex queue, [1 + t] workgroup tier t
+    long long *tier[3];          // vertices of the workgroup tiers (counts ovf_n[t])
     unsigned long long *ovf_n;
     long long ovf_cap;
     unsigned long long *bytes;   // W_alg accumulator of this launch
@@ -802,9 +802,10 @@ __device__ __forceinline__ int row_popc(RP rows, int p, int w) {
 // worker k takes pairs first, first + step, ..., so lanes share the work
 // evenly however uneven the rows are. For a pair: t = M_p & M_q gives #K4
 // += popc(t); each r in t adds popc(t & M_r) to #K5; each s in t & M_r adds
-// popc(t & M_r & M_s) to #K6. MW (words held in registers) must cover the
-// rows; MW == 0 reads every word from rows as needed (any width).
-template <int MW, class RP>
+// popc(t & M_r & M_s) to #K6. Words are read from the rows as needed (no
+// per-lane row copies: that keeps register use, and so occupancy, low),
+// each loop stopping at the last word that can hold bits.
+template <class RP>
 __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off, int P, int first, int step,
                           unsigned long long *acc, unsigned long long &bytes) {
     for (int k = first; k < P; k += step) {
@@ -826,86 +827,37 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
             j -= c;
         }
         const int wq = (q >> 5) + 1;
-        if (a.account) bytes += 8ull * w;
+        if (a.account) bytes += 8ull * wq;
         const long long pw = (long long)p * w, qw = (long long)q * w;
-        if constexpr (MW > 0) {
-            unsigned t[MW];
-            unsigned c4 = 0;
-#pragma unroll
-            for (int u = 0; u < MW; u++) {
-                t[u] = u < wq ? (rows[pw + u] & rows[qw + u]) : 0u;
-                c4 += __popc(t[u]);
-            }
-            acc[4] += c4;
-            if (a.kmax < 5) continue;
-            for (int rw = 0; rw < wq; rw++) {
-                unsigned br = t[rw];
-                while (br) {
-                    const int r = rw * 32 + __ffs(br) - 1;
-                    br &= br - 1;
-                    const int wr = (r >> 5) + 1;
-                    const long long rr = (long long)r * w;
-                    if (a.account) bytes += 8ull * w;
-                    if (a.kmax < 6) {
-                        unsigned c5 = 0;
-#pragma unroll
-                        for (int u = 0; u < MW; u++)
-                            if (u < wr) c5 += __popc(t[u] & rows[rr + u]);
-                        acc[5] += c5;
-                        continue;
-                    }
-                    unsigned t2[MW];
-                    unsigned c5 = 0;
-#pragma unroll
-                    for (int u = 0; u < MW; u++) {
-                        t2[u] = u < wr ? (t[u] & rows[rr + u]) : 0u;
-                        c5 += __popc(t2[u]);
-                    }
-                    acc[5] += c5;
-                    unsigned c6 = 0;
-                    for (int sw = 0; sw < wr; sw++) {
-                        unsigned bs = t2[sw];
-                        while (bs) {
-                            const int sx = sw * 32 + __ffs(bs) - 1;
-                            bs &= bs - 1;
-                            const int wsx = (sx >> 5) + 1;
-                            const long long ss = (long long)sx * w;
-                            if (a.account) bytes += 8ull * w;
-#pragma unroll
-                            for (int u = 0; u < MW; u++)
-                                if (u < wsx) c6 += __popc(t2[u] & rows[ss + u]);
-                        }
-                    }
-                    acc[6] += c6;
-                }
-            }
-        } else {
-            for (int u = 0; u < wq; u++) acc[4] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u]);
-            if (a.kmax < 5) continue;
-            for (int rw = 0; rw < wq; rw++) {
-                unsigned br = rows[pw + rw] & rows[qw + rw];
-                while (br) {
-                    const int r = rw * 32 + __ffs(br) - 1;
-                    br &= br - 1;
-                    const int wr = (r >> 5) + 1;
-                    const long long rr = (long long)r * w;
-                    if (a.account) bytes += 8ull * w;
-                    for (int u = 0; u < wr; u++)
-                        acc[5] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
-                    if (a.kmax < 6) continue;
-                    for (int sw = 0; sw < wr; sw++) {
-                        unsigned bs = rows[pw + sw] & rows[qw + sw] & rows[rr + sw];
-                        while (bs) {
-                            const int sx = sw * 32 + __ffs(bs) - 1;
-                            bs &= bs - 1;
-                            const long long ss = (long long)sx * w;
-                            if (a.account) bytes += 8ull * w;
-                            for (int u = 0; u <= (sx >> 5); u++)
-                                acc[6] += (unsigned long long)__popc(rows[pw + u] & rows[qw + u] & rows[rr + u] &
-                                                                     rows[ss + u]);
-                        }
+        unsigned c4 = 0;
+        for (int u = 0; u < wq; u++) c4 += __popc(rows[pw + u] & rows[qw + u]);
+        acc[4] += c4;
+        if (a.kmax < 5) continue;
+        for (int rw = 0; rw < wq; rw++) {
+            unsigned br = rows[pw + rw] & rows[qw + rw];
+            while (br) {
+                const int r = rw * 32 + __ffs(br) - 1;
+                br &= br - 1;
+                const int wr = (r >> 5) + 1;
+                const long long rr = (long long)r * w;
+                if (a.account) bytes += 8ull * wr;
+                unsigned c5 = 0;
+                for (int u = 0; u < wr; u++) c5 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
+                acc[5] += c5;
+                if (a.kmax < 6) continue;
+                unsigned c6 = 0;
+                for (int sw = 0; sw < wr; sw++) {
+                    unsigned bs = rows[pw + sw] & rows[qw + sw] & rows[rr + sw];
+                    while (bs) {
+                        const int sx = sw * 32 + __ffs(bs) - 1;
+                        bs &= bs - 1;
+                        const long long ss = (long long)sx * w;
+                        if (a.account) bytes += 8ull * ((sx >> 5) + 1);
+                        for (int u = 0; u <= (sx >> 5); u++)
+                            c6 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u] & rows[ss + u]);
                     }
                 }
+                acc[6] += c6;
             }
         }
     }
@@ -943,9 +895,9 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             if (d < 2) continue;
             if (d > a.cap) {
                 if (ln == 0) {
-                    long long *list = d <= 32 * CMIDW ? a.ovf : a.ovf2;
-                    const unsigned long long idx = atomicAdd(&a.ovf_n[d <= 32 * CMIDW ? 0 : 1], 1ull);
-                    if ((long long)idx < a.ovf_cap) list[idx] = v0; else atomicOr(a.err, 4u);
+                    const int t = d <= 32 * CMIDW ? 0 : (d <= 32 * CBIGW ? 1 : 2);
+                    const unsigned long long idx = atomicAdd(&a.ovf_n[t], 1ull);
+                    if ((long long)idx < a.ovf_cap) a.tier[t][idx] = v0; else atomicOr(a.err, 4u);
                 }
                 continue;
             }
@@ -963,7 +915,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             for (int pb = 0; pb < d; pb += W) {
                 const int p = pb + ln;
                 const int c = p < d ? row_popc(rows, p, w) : 0;
-                if (a.account && p < d) bytes += 4ull * w;
+                if (a.account && p < d) bytes += 4ull * ((p >> 5) + 1);
                 int tot = 0;
                 const int x = excl_scan(c, &tot);
                 if (p < d) off[p] = run + x;
@@ -974,23 +926,24 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                 acc[3] += (unsigned long long)run;
             }
             wsync();
-            if (a.kmax >= 4) vtx_pairs<CMAXW>(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
+            if (a.kmax >= 4) vtx_pairs(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
             wsync();
         }
     }
     clique_flush(a, acc, bytes, ln);
 }
 
-// Block tiers: the NW waves of a workgroup share one vertex. LDS: rows of
-// vertices with d <= 32 * CMIDW, 16 words wide; HBM: any d up to hcap, rows
-// in the workgroup's scratch. The waves split the rows between them for the
-// build and for the count.
-template <bool LDS, int NW>
+// Workgroup tiers: the NW waves of a workgroup share one vertex of tier
+// list TIER. Rows MW words wide (d <= 32 * MW) sit in LDS or, without LDS,
+// in the workgroup's HBM scratch; MW == 0 (any d up to hcap) reads every
+// word from HBM as needed. The waves split the rows between them for the
+// build and take pairs round-robin for the count.
+template <int MW, bool LDS, int TIER, int NW>
 __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
-    constexpr int lcap = 32 * CMIDW;
-    // LDS tier: the list (then the pair offsets), then the rows
+    constexpr int lcap = 32 * MW;
+    // LDS: the list (then the pair offsets), then the rows
     int *Ls = reinterpret_cast<int *>(smem);
     unsigned *rows_l = reinterpret_cast<unsigned *>(Ls + (LDS ? lcap + 4 : 0));
     __shared__ int vsh;
@@ -1001,9 +954,9 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;
     for (;;) {
         if (threadIdx.x == 0) {
-            const unsigned long long r = atomicAdd(&a.queue[LDS ? 1 : 2], 1ull);
-            const unsigned long long cnt = a.ovf_n[LDS ? 0 : 1];
-            vsh = (r < cnt && (long long)r < a.ovf_cap) ? (int)(LDS ? a.ovf[r] : a.ovf2[r]) : -1;
+            const unsigned long long r = atomicAdd(&a.queue[1 + TIER], 1ull);
+            const unsigned long long cnt = a.ovf_n[TIER];
+            vsh = (r < cnt && (long long)r < a.ovf_cap) ? (int)a.tier[TIER][r] : -1;
         }
         __syncthreads();
         const int v0 = vsh;
@@ -1011,7 +964,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
         const int d = a.split[v0];
         const long long rb = a.off[v0];
         const int w = (d + 31) >> 5;
-        if (d > (LDS ? lcap : a.hcap)) {  // cannot happen: lists are split by d
+        if (d > (MW > 0 ? lcap : a.hcap) || d > a.hcap) {  // cannot happen: lists are split by d
             if (threadIdx.x == 0) atomicOr(a.err, 8u);
         } else {
             if (LDS)
@@ -1031,7 +984,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             for (int pb = 0; pb < d; pb += 64 * NW) {
                 const int p = pb + (int)threadIdx.x;
                 const int c = p < d ? row_popc((const unsigned *)rows, p, w) : 0;
-                if (a.account && p < d) bytes += 4ull * w;
+                if (a.account && p < d) bytes += 4ull * ((p >> 5) + 1);
                 int tot = 0;
                 const int x = excl_scan(c, &tot);
                 if (ln == 0) wtot[wid] = tot;
@@ -1053,8 +1006,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             if (!LDS) __threadfence();
             __syncthreads();
             if (a.kmax >= 4)
-                vtx_pairs<LDS ? CMIDW : 0>(a, (const unsigned *)rows, d, w, off, run, (int)threadIdx.x, 64 * NW, acc,
-                                           bytes);
+                vtx_pairs(a, (const unsigned *)rows, d, w, off, run, (int)threadIdx.x, 64 * NW, acc, bytes);
         }
         __syncthreads();
     }
@@ -1180,7 +1132,7 @@ struct PlanDevice {
     unsigned long long *counts = nullptr, *queue = nullptr, *ovf_n = nullptr, *bytes = nullptr;
     unsigned int *err = nullptr;
     long long *ovf = nullptr;
-    long long *ovf2 = nullptr;  // clique edges left for the HBM-row kernel
+    long long *tier[3] = {};  // clique vertices of the workgroup tiers
     long long ovf_cap = 0;
     int *gstore = nullptr;
     size_t gstore_bytes = 0;
@@ -1203,7 +1155,8 @@ void destroy_plan_device(PlanDevice *d) {
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
-                    (void *)d->err, (void *)d->ovf, (void *)d->ovf2, (void *)d->gstore, (void *)d->grows,
+                    (void *)d->err, (void *)d->ovf, (void *)d->tier[0], (void *)d->tier[1], (void *)d->tier[2],
+                    (void *)d->gstore, (void *)d->grows,
                     (void *)d->goff})
         if (p) (void)hipFree(p);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -1291,9 +1244,11 @@ struct Launch {
     gatk::CArgs ca;
     size_t clique_lds = 0;
     int clique_wave_bytes = 0;
-    dim3 mid_grid;
+    dim3 mid_grid;      // workgroup tier 0 (rows in LDS)
     size_t mid_lds = 0;
-    dim3 big_grid;
+    dim3 t1_grid;       // workgroup tier 1 (LDS when t1_lds > 0, else HBM)
+    size_t t1_lds = 0;
+    dim3 big_grid;      // workgroup tier 2 (HBM)
 };
 
 static void fill_defaults(gat_kernel_config *c) {
@@ -1327,8 +1282,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         HIPCHECK(hipMemcpy(d->plan, &pd, sizeof(GatPlanDev), hipMemcpyHostToDevice));
         HIPCHECK(hipMalloc(&d->counts, sizeof(unsigned long long) * GATP_MAX_TERMS));
         HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 4));
-        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 2));
-        HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 2));
+        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 4));
+        HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 4));
         HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
         for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
     }
@@ -1348,15 +1303,18 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     }
     if (cfg.vertex_store == 2 && !clique)
         GAT_FAIL(GAT_EINFEASIBLE, "bitmap vertex store needs a plan of cliques K3..K6 in unique or labeled mode");
-    const long long ovf_cap = clique ? g.m + 1 : (g.m + chunk - 1) / chunk + 1;
+    const long long ovf_cap = clique ? 1 : (g.m + chunk - 1) / chunk + 1;
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
-        if (d->ovf2) HIPCHECK(hipFree(d->ovf2));
-        d->ovf = d->ovf2 = nullptr;
+        for (auto &t : d->tier)
+            if (t) HIPCHECK(hipFree(t));
+        d->ovf = nullptr;
+        for (auto &t : d->tier) t = nullptr;
         HIPCHECK(hipMalloc(&d->ovf, sizeof(long long) * ovf_cap));
         d->ovf_cap = ovf_cap;
     }
-    if (clique && !d->ovf2) HIPCHECK(hipMalloc(&d->ovf2, sizeof(long long) * d->ovf_cap));
+    if (clique && !d->tier[0])
+        for (auto &t : d->tier) HIPCHECK(hipMalloc(&t, sizeof(long long) * (size_t)(g.n + 1)));
     const int wave_bytes = int((sizeof(gatk::WaveState) + size_t(pd.n_store) * cfg.slot_capacity * 4 + 15) / 16 * 16);
     const size_t lds = size_t(wave_bytes) * cfg.block_waves;
     if (lds > di.lds_per_block)
@@ -1414,10 +1372,9 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         for (int k = 0; k < 8; k++) c.pat_of_k[k] = p->cp.clique_pattern[k];
         c.kmax = 31 - __builtin_clz(p->cp.clique_ks);
         c.queue = a.queue;
-        c.ovf = d->ovf;
-        c.ovf2 = d->ovf2;
+        for (int t = 0; t < 3; t++) c.tier[t] = d->tier[t];
         c.ovf_n = a.ovf_n;
-        c.ovf_cap = d->ovf_cap;
+        c.ovf_cap = g.n + 1;
         c.bytes = a.bytes;
         c.err = a.err;
         c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
@@ -1459,6 +1416,9 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique block tier");
         L->mid_grid = dim3(std::max(1, (int)(di.lds_per_cu / L->mid_lds)) * di.cus);
         L->big_grid = dim3(std::min(d->big_waves, 2 * di.cus));  // one scratch area per workgroup
+        const size_t t1 = (size_t)4 * (32 * gatk::CBIGW + 4) + (size_t)4 * 32 * gatk::CBIGW * gatk::CBIGW;
+        L->t1_lds = t1 <= di.lds_per_block ? t1 : 0;
+        L->t1_grid = L->t1_lds ? dim3(std::max(1, (int)(di.lds_per_cu / t1)) * di.cus) : L->big_grid;
     }
     return GAT_OK;
 }
@@ -1468,8 +1428,8 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     hipStream_t s = d->stream;
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->counts, (long long)GATP_MAX_TERMS, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 4ll, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 2ll, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->bytes, 2ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 4ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->bytes, 4ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
     gatk::KArgs a = L.a, ao = L.ao;
     a.account = ao.account = account ? 1 : 0;
@@ -1484,13 +1444,22 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
-        c.bytes = d->bytes + 1;  // later tiers: their own W_alg counter
-        hipLaunchKernelGGL((gatk::clique_block_kernel<true, gatk::CBLOCKW>), L.mid_grid, dim3(64 * gatk::CBLOCKW),
-                           L.mid_lds, s, c);
-        hipLaunchKernelGGL((gatk::clique_block_kernel<false, gatk::CBLOCKW>), L.big_grid, dim3(64 * gatk::CBLOCKW), 0,
-                           s, c);
+        c.bytes = d->bytes + 1;  // workgroup tier 0: its own W_alg counter
+        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW>), L.mid_grid,
+                           dim3(64 * gatk::CBLOCKW), L.mid_lds, s, c);
+        if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
+        c.bytes = d->bytes + 2;  // tiers 1 and 2
+        if (L.t1_lds)
+            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW>), L.t1_grid,
+                               dim3(64 * gatk::CBLOCKW), L.t1_lds, s, c);
+        else
+            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW>), L.t1_grid,
+                               dim3(64 * gatk::CBLOCKW), 0, s, c);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW>), L.big_grid,
+                           dim3(64 * gatk::CBLOCKW), 0, s, c);
     } else {
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+        if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[2], s));
     HIPCHECK(hipGetLastError());
@@ -1500,7 +1469,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
 static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf[2] = {0, 0}, bytes[2] = {0, 0};
+    unsigned long long raw[GATP_MAX_TERMS], ovf[3] = {0, 0, 0}, bytes[3] = {0, 0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
@@ -1518,15 +1487,18 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
     if (st) {
         std::memset(st, 0, sizeof(*st));
         if (timed) {
-            float ms = 0, ms_main = 0;
+            float ms = 0, ms_main = 0, ms_second = 0;
             HIPCHECK(hipEventElapsedTime(&ms, d->ev[0], d->ev[2]));
             HIPCHECK(hipEventElapsedTime(&ms_main, d->ev[0], d->ev[1]));
+            HIPCHECK(hipEventElapsedTime(&ms_second, d->ev[1], d->ev[3]));
             st->seconds = ms / 1000.0;
             st->main_kernel_ms = ms_main;
+            st->second_kernel_ms = ms_second;
         }
-        st->overflow_tasks = ovf[0] + ovf[1];
-        st->alg_bytes = bytes[0] + bytes[1];
+        st->overflow_tasks = ovf[0] + ovf[1] + ovf[2];
+        st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
+        st->alg_bytes_second = bytes[1];
         st->tasks = (uint64_t)p->graph->m;
         st->grid_blocks = (int)L.grid.x;
         st->block_threads = (int)L.block.x;

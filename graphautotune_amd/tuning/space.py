@@ -22,7 +22,8 @@ SPACE = [
     Param("block_waves", [1, 2, 4], 1, "ThreadsPerBlock (details.py:50-52) / 64"),
     Param("waves_per_cu", [4, 8, 12, 16], 8, "NumBlocks (details.py:45-47), per CU"),
     Param("queue_chunk", [1, 4, 16, 64], 16, "edges per work-queue grab (prefix chunk)"),
-    Param("slot_capacity", [128, 256, 512, 1024], 1024, "LDS set-buffer entries"),
+    Param("slot_capacity", [32, 64, 128, 256, 512, 1024], 1024,
+          "LDS set-buffer entries (array store); per-wave row limit d <= min(256, value) (bitmap store)"),
     Param("intersect_strategy", [0, 1, 2], 0, "auto / stream-and-search / stream-shorter"),
     Param("gallop_ratio", [4, 32, 256], 32, "skew ratio that switches to galloping"),
     Param("search_linear_threshold", [0, 4, 16, 64, INT_MAX], 0, "SetSearchType (details.py:11-13)"),

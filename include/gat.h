@@ -134,12 +134,16 @@ typedef struct gat_kernel_config {
 /* Per-call statistics filled by gat_count (any pointer may be NULL). */
 typedef struct gat_count_stats {
     double seconds;           /* device time of the count (hipEvents)        */
-    double main_kernel_ms;    /* average duration of the main engine launch  */
+    double main_kernel_ms;    /* duration of the main engine launch          */
+    double second_kernel_ms;  /* duration of the second launch: the HBM
+                                 overflow launch (array store) or workgroup
+                                 tier 0 (bitmap store)                        */
     uint64_t overflow_tasks;  /* array store: tasks re-run with HBM-resident
                                  prefix sets; bitmap store: vertices passed to
                                  the workgroup tiers                          */
     uint64_t alg_bytes;       /* W_alg of the whole count (GAT_FLAG_ACCOUNT)  */
     uint64_t alg_bytes_main;  /* W_alg of the main launch alone              */
+    uint64_t alg_bytes_second;/* W_alg of the second launch alone            */
     uint64_t tasks;           /* directed edges in the task space            */
     int32_t grid_blocks;      /* launched workgroups                         */
     int32_t block_threads;

@@ -36,6 +36,9 @@ def lib():
         L.orc_clique_walg_sample.restype = C.c_uint64
         L.orc_clique_bitmap_walg.argtypes = [C.c_int32, P, P, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_clique_bitmap_walg.restype = C.c_uint64
+        L.orc_clique_bitmap_walg_sample.argtypes = [C.c_int32, P, P, C.c_int, C.c_int32, C.c_int32,
+                                                    C.POINTER(C.c_uint64)]
+        L.orc_clique_bitmap_walg_sample.restype = C.c_uint64
         L.orc_relabel.argtypes = [C.c_int32, P, P, P, P]
         L.orc_relabel.restype = None
         _lib = L
@@ -121,4 +124,14 @@ def clique_bitmap_walg(off_relabeled, adj_relabeled, kmax):
     off, adj, n = _csr(off_relabeled, adj_relabeled)
     w = C.c_uint64()
     c = lib().orc_clique_bitmap_walg(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, kmax, C.byref(w))
+    return int(c), int(w.value)
+
+
+def clique_bitmap_walg_sample(off_relabeled, adj_relabeled, kmax, stride, first=0):
+    """(#K_kmax with top vertex in first, first + stride, ..., W_alg of the
+    bitmap store over those top vertices)."""
+    off, adj, n = _csr(off_relabeled, adj_relabeled)
+    w = C.c_uint64()
+    c = lib().orc_clique_bitmap_walg_sample(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, kmax,
+                                            stride, first, C.byref(w))
     return int(c), int(w.value)

@@ -1041,31 +1041,38 @@ void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off
 }
 
 /* Algorithmic bytes of the bitmap clique store (cliques K3..kmax, kmax <= 6)
- * on a descending-degree graph. Per vertex v0 with d = |N+(v0)| >= 2
- * (L = N+(v0), w = ceil(d/32)): 4d + 16 to read L; per position p of L,
- * 4(|N+(L[p])| + p) + 16 to build row M_p = {q < p : L[q] in N+(L[p])} and
- * 4w to count it; 8w per pair (p,q), q in M_p (when kmax >= 4), per triple
- * (kmax >= 5) and per quadruple (kmax >= 6) of mutually adjacent positions.
+ * on a descending-degree graph. Per vertex v0 with d = |N+(v0)| >= 2 (L =
+ * N+(v0)): 4d + 16 to read L; per position p of L, 4(|N+(L[p])| + p) + 16
+ * to build row M_p = {q < p : L[q] in N+(L[p])} and 4 w(p) to count it,
+ * where w(x) = floor(x/32) + 1 is the number of words that can hold bits
+ * below x. For mutually adjacent positions p > q > r > s: 8 w(q) per pair
+ * (p,q) (when kmax >= 4), 8 w(r) per triple (kmax >= 5) and 8 w(s) per
+ * quadruple (kmax >= 6): the words of the two rows each step intersects.
  * Every clique with top vertex v0 is v0 plus a clique of these rows, so
  * #K3 = #pairs, #K4 = #triples, ... Returns #K_kmax. */
 uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *adj, int kmax, uint64_t *walg) {
+    return orc_clique_bitmap_walg_sample(n, off, adj, kmax, 1, 0, walg);
+}
+
+/* The same over the top vertices v0 = first, first + stride, ... only. */
+uint64_t orc_clique_bitmap_walg_sample(int32_t n, const int64_t *off, const int32_t *adj, int kmax, int32_t stride,
+                                       int32_t first, uint64_t *walg) {
     int32_t v0, maxd = 0, v;
     uint64_t bytes = 0, cnt[7] = {0};
     unsigned char *M; /* M[p*d+q] = 1 iff q < p and L[q] in N+(L[p]) */
     for (v = 0; v < n; v++)
         if (off[v + 1] - off[v] > maxd) maxd = (int32_t)(off[v + 1] - off[v]);
     M = (unsigned char *)malloc((size_t)(maxd + 1) * (maxd + 1));
-    for (v0 = 0; v0 < n; v0++) {
+    for (v0 = first; v0 < n; v0 += stride) {
         const int32_t *L = adj + off[v0];
         const int64_t d = lower_bound_i32(L, off[v0 + 1] - off[v0], v0);
-        int64_t w, p, q, r, t, u;
+        int64_t p, q, r, t, u;
         if (d < 2) continue;
-        w = (d + 31) / 32;
         bytes += 4ull * (uint64_t)d + 16ull;
         for (p = 0; p < d; p++) {
             const int32_t x = L[p];
             const int64_t xl = lower_bound_i32(adj + off[x], off[x + 1] - off[x], x);
-            bytes += 4ull * (uint64_t)(xl + p) + 16ull + 4ull * (uint64_t)w;
+            bytes += 4ull * (uint64_t)(xl + p) + 16ull + 4ull * (uint64_t)(p / 32 + 1);
             for (q = 0; q < d; q++) M[p * d + q] = q < p && in_sorted(adj + off[x], xl, L[q]);
         }
         for (p = 0; p < d; p++)
@@ -1073,17 +1080,17 @@ uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *ad
                 if (!M[p * d + q]) continue;
                 cnt[3]++;
                 if (kmax < 4) continue;
-                bytes += 8ull * (uint64_t)w;
+                bytes += 8ull * (uint64_t)(q / 32 + 1);
                 for (r = 0; r < q; r++) {
                     if (!(M[p * d + r] && M[q * d + r])) continue;
                     cnt[4]++;
                     if (kmax < 5) continue;
-                    bytes += 8ull * (uint64_t)w;
+                    bytes += 8ull * (uint64_t)(r / 32 + 1);
                     for (t = 0; t < r; t++) {
                         if (!(M[p * d + t] && M[q * d + t] && M[r * d + t])) continue;
                         cnt[5]++;
                         if (kmax < 6) continue;
-                        bytes += 8ull * (uint64_t)w;
+                        bytes += 8ull * (uint64_t)(t / 32 + 1);
                         for (u = 0; u < t; u++)
                             if (M[p * d + u] && M[q * d + u] && M[r * d + u] && M[t * d + u]) cnt[6]++;
                     }

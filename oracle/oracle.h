@@ -51,6 +51,9 @@ uint64_t orc_clique_walg_sample(int32_t n, const int64_t *off, const int32_t *ad
                                 int32_t first, uint64_t *walg);
 /* Algorithmic bytes of the bitmap clique path for cliques K3..kmax. */
 uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *adj, int kmax, uint64_t *walg);
+/* The same restricted to top vertices v0 = first, first + stride, ... */
+uint64_t orc_clique_bitmap_walg_sample(int32_t n, const int64_t *off, const int32_t *adj, int kmax, int32_t stride,
+                                       int32_t first, uint64_t *walg);
 /* Relabel to descending degree (ties: ascending id). Output arrays must
  * hold n+1 offsets and off[n] entries. */
 void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off_out, int32_t *adj_out);

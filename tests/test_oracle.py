@@ -254,9 +254,10 @@ def test_relabel_and_clique_walg(orc):
 def test_clique_bitmap_walg_definition(orc):
     """Counts of the bitmap-store restatement equal the brute-force unique
     counts, and its W_alg equals the definition recomputed here: per v0 with
-    d = |N+(v0)| >= 2, 4d + 16; per position p, 4(|N+(L[p])| + p) + 16 + 4w;
-    8w per pair, triple and quadruple of mutually adjacent positions that
-    the deeper levels visit (w = ceil(d/32))."""
+    d = |N+(v0)| >= 2, 4d + 16; per position p, 4(|N+(L[p])| + p) + 16 +
+    4 w(p); for each pair, triple and quadruple of mutually adjacent
+    positions that the deeper levels visit, 8 w(smallest position), with
+    w(x) = x // 32 + 1."""
     off, adj = random_graph(45, 0.3, 7)
     ro, ra = orc.relabel(off, adj)
     n = len(ro) - 1
@@ -270,8 +271,7 @@ def test_clique_bitmap_walg_definition(orc):
             d = len(L)
             if d < 2:
                 continue
-            wd = (d + 31) // 32
-            tot += 4 * d + 16 + sum(4 * (len(up[x]) + p) + 16 + 4 * wd for p, x in enumerate(L))
+            tot += 4 * d + 16 + sum(4 * (len(up[x]) + p) + 16 + 4 * (p // 32 + 1) for p, x in enumerate(L))
             adjL = {(p, q) for p in range(d) for q in range(p) if L[q] in set(up[L[p]])}
             # cliques of size j among positions, visited at depth j (j = 2, 3, 4)
             for size in (2, 3, 4):
@@ -279,5 +279,14 @@ def test_clique_bitmap_walg_definition(orc):
                     break
                 for combo in itertools.combinations(range(d), size):
                     if all((b, a) in adjL for a, b in itertools.combinations(combo, 2)):
-                        tot += 8 * wd
+                        tot += 8 * (min(combo) // 32 + 1)
         assert w == tot, (k, w, tot)
+
+
+def test_clique_bitmap_walg_sample_partitions(orc):
+    off, adj = random_graph(80, 0.2, 11)
+    ro, ra = orc.relabel(off, adj)
+    for k in (3, 5):
+        full = orc.clique_bitmap_walg(ro, ra, k)
+        parts = [orc.clique_bitmap_walg_sample(ro, ra, k, 4, f) for f in range(4)]
+        assert (sum(p[0] for p in parts), sum(p[1] for p in parts)) == full

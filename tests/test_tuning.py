@@ -17,7 +17,7 @@ def test_space_matches_reference_values():
     # tuning/tuning/config/details.py:11-13 (SetSearchType) kept verbatim
     assert sp.BY_NAME["search_linear_threshold"].values == [0, 4, 16, 64, 2 ** 31 - 1]
     full = sp.ConfigSpace()
-    assert full.size() == 3 * 4 * 4 * 4 * 3 * 3 * 5 * 2
+    assert full.size() == 3 * 4 * 4 * 6 * 3 * 3 * 5 * 2
     assert len(list(sp.ConfigSpace(["block_waves", "queue_chunk"]).all())) == 12
 
 
