@@ -699,7 +699,9 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
 constexpr int CMAXW = 8;   // main tier: d <= 256
 constexpr int CMIDW = 16;  // workgroup tier 0: d <= 512, rows in LDS
 constexpr int CBIGW = 32;  // workgroup tier 1: d <= 1024, rows in LDS when it fits, else HBM
-constexpr int CBLOCKW = 8; // waves per workgroup in the workgroup tiers
+constexpr int CSPLIT = 4;     // work items per vertex in tiers 1 and 2
+constexpr int CBLOCKW0 = 8;   // waves per workgroup, tier 0
+constexpr int CBLOCKW1 = 16;  // waves per workgroup, tiers 1 and 2 (few, heavy vertices)
 
 struct CArgs {
     const long long *off;
@@ -895,9 +897,17 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             if (d < 2) continue;
             if (d > a.cap) {
                 if (ln == 0) {
+                    // heavy vertices (tiers 1, 2) are split into CSPLIT work
+                    // items that share out the pairs
                     const int t = d <= 32 * CMIDW ? 0 : (d <= 32 * CBIGW ? 1 : 2);
-                    const unsigned long long idx = atomicAdd(&a.ovf_n[t], 1ull);
-                    if ((long long)idx < a.ovf_cap) a.tier[t][idx] = v0; else atomicOr(a.err, 4u);
+                    const int parts = t ? CSPLIT : 1;
+                    const unsigned long long idx = atomicAdd(&a.ovf_n[t], (unsigned long long)parts);
+                    if ((long long)(idx + parts) <= a.ovf_cap) {
+                        // tier 0 items are vertex ids; later tiers (vertex, part) pairs
+                        for (int k = 0; k < parts; k++) a.tier[t][idx + k] = t ? (long long)v0 * CSPLIT + k : v0;
+                    } else {
+                        atomicOr(a.err, 4u);
+                    }
                 }
                 continue;
             }
@@ -946,7 +956,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     // LDS: the list (then the pair offsets), then the rows
     int *Ls = reinterpret_cast<int *>(smem);
     unsigned *rows_l = reinterpret_cast<unsigned *>(Ls + (LDS ? lcap + 4 : 0));
-    __shared__ int vsh;
+    __shared__ long long vsh;
     __shared__ int wtot[NW];
     const int hws = (a.hcap + 31) / 32;
     unsigned *rows_g = a.grows + (long long)blockIdx.x * a.hcap * hws;
@@ -956,11 +966,14 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
         if (threadIdx.x == 0) {
             const unsigned long long r = atomicAdd(&a.queue[1 + TIER], 1ull);
             const unsigned long long cnt = a.ovf_n[TIER];
-            vsh = (r < cnt && (long long)r < a.ovf_cap) ? (int)a.tier[TIER][r] : -1;
+            vsh = (r < cnt && (long long)r < a.ovf_cap) ? a.tier[TIER][r] : -1;
         }
         __syncthreads();
-        const int v0 = vsh;
-        if (v0 < 0) break;
+        if (vsh < 0) break;
+        // tier 0 items are vertices; later tiers carry (vertex, part)
+        const int v0 = (int)(TIER ? vsh / CSPLIT : vsh);
+        const int part = (int)(TIER ? vsh % CSPLIT : 0);
+        const int parts = TIER ? CSPLIT : 1;
         const int d = a.split[v0];
         const long long rb = a.off[v0];
         const int w = (d + 31) >> 5;
@@ -971,11 +984,13 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
                 for (int k = threadIdx.x; k < d; k += 64 * NW) Ls[k] = a.adj[rb + k];
             unsigned *rows = LDS ? rows_l : rows_g;
             for (long long k = threadIdx.x; k < (long long)d * w; k += 64 * NW) rows[k] = 0u;
-            if (a.account && threadIdx.x == 0) bytes += 4ull * d + 16ull;
+            // every part builds the rows; W_alg charges one build
+            unsigned long long build_bytes = 0;
+            if (a.account && threadIdx.x == 0) build_bytes += 4ull * d + 16ull;
             if (!LDS) __threadfence();
             __syncthreads();
             const int *L = LDS ? Ls : a.adj + rb;
-            vtx_rows(a, L, d, rows, w, wid * W, W * NW, ln, bytes);
+            vtx_rows(a, L, d, rows, w, wid * W, W * NW, ln, build_bytes);
             if (!LDS) __threadfence();
             __syncthreads();
             // pair offsets: block-wide exclusive prefix of popc(M_p)
@@ -984,7 +999,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             for (int pb = 0; pb < d; pb += 64 * NW) {
                 const int p = pb + (int)threadIdx.x;
                 const int c = p < d ? row_popc((const unsigned *)rows, p, w) : 0;
-                if (a.account && p < d) bytes += 4ull * ((p >> 5) + 1);
+                if (a.account && p < d) build_bytes += 4ull * ((p >> 5) + 1);
                 int tot = 0;
                 const int x = excl_scan(c, &tot);
                 if (ln == 0) wtot[wid] = tot;
@@ -1001,12 +1016,14 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             }
             if (threadIdx.x == 0) {
                 off[d] = run;
-                acc[3] += (unsigned long long)run;
+                if (part == 0) acc[3] += (unsigned long long)run;
             }
+            if (part == 0) bytes += build_bytes;
             if (!LDS) __threadfence();
             __syncthreads();
             if (a.kmax >= 4)
-                vtx_pairs(a, (const unsigned *)rows, d, w, off, run, (int)threadIdx.x, 64 * NW, acc, bytes);
+                vtx_pairs(a, (const unsigned *)rows, d, w, off, run, part * 64 * NW + (int)threadIdx.x,
+                          64 * NW * parts, acc, bytes);
         }
         __syncthreads();
     }
@@ -1314,7 +1331,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         d->ovf_cap = ovf_cap;
     }
     if (clique && !d->tier[0])
-        for (auto &t : d->tier) HIPCHECK(hipMalloc(&t, sizeof(long long) * (size_t)(g.n + 1)));
+        for (auto &t : d->tier) HIPCHECK(hipMalloc(&t, sizeof(long long) * (size_t)(g.n + 1) * gatk::CSPLIT));
     const int wave_bytes = int((sizeof(gatk::WaveState) + size_t(pd.n_store) * cfg.slot_capacity * 4 + 15) / 16 * 16);
     const size_t lds = size_t(wave_bytes) * cfg.block_waves;
     if (lds > di.lds_per_block)
@@ -1374,7 +1391,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         c.queue = a.queue;
         for (int t = 0; t < 3; t++) c.tier[t] = d->tier[t];
         c.ovf_n = a.ovf_n;
-        c.ovf_cap = g.n + 1;
+        c.ovf_cap = (long long)(g.n + 1) * gatk::CSPLIT;
         c.bytes = a.bytes;
         c.err = a.err;
         c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
@@ -1445,18 +1462,18 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
         c.bytes = d->bytes + 1;  // workgroup tier 0: its own W_alg counter
-        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW>), L.mid_grid,
-                           dim3(64 * gatk::CBLOCKW), L.mid_lds, s, c);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW0>), L.mid_grid,
+                           dim3(64 * gatk::CBLOCKW0), L.mid_lds, s, c);
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
         c.bytes = d->bytes + 2;  // tiers 1 and 2
         if (L.t1_lds)
-            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW>), L.t1_grid,
-                               dim3(64 * gatk::CBLOCKW), L.t1_lds, s, c);
+            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW1>), L.t1_grid,
+                               dim3(64 * gatk::CBLOCKW1), L.t1_lds, s, c);
         else
-            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW>), L.t1_grid,
-                               dim3(64 * gatk::CBLOCKW), 0, s, c);
-        hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW>), L.big_grid,
-                           dim3(64 * gatk::CBLOCKW), 0, s, c);
+            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW1>), L.t1_grid,
+                               dim3(64 * gatk::CBLOCKW1), 0, s, c);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1>), L.big_grid,
+                           dim3(64 * gatk::CBLOCKW1), 0, s, c);
     } else {
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
@@ -1495,7 +1512,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->main_kernel_ms = ms_main;
             st->second_kernel_ms = ms_second;
         }
-        st->overflow_tasks = ovf[0] + ovf[1] + ovf[2];
+        st->overflow_tasks = ovf[0] + (ovf[1] + ovf[2]) / gatk::CSPLIT;  // clique tiers 1, 2: CSPLIT parts each
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
         st->alg_bytes_second = bytes[1];
