@@ -799,6 +799,15 @@ __device__ __forceinline__ int row_popc(RP rows, int p, int w) {
     return c;
 }
 
+// t[i] for a run-time i < N, by selects (constant indexes keep t in registers).
+template <int N>
+__device__ __forceinline__ unsigned pick(const unsigned (&t)[N], int i) {
+    unsigned v = t[0];
+#pragma unroll
+    for (int k = 1; k < N; k++) v = i == k ? t[k] : v;
+    return v;
+}
+
 // Counts at and below the pair level. The pairs (p, q), q in M_p, are
 // numbered through off[p] = sum of popc(M_p') over p' < p (off[d] = P), and
 // worker k takes pairs first, first + step, ..., so lanes share the work
@@ -831,12 +840,21 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
         const int wq = (q >> 5) + 1;
         if (a.account) bytes += 8ull * wq;
         const long long pw = (long long)p * w, qw = (long long)q * w;
+        // the first TC words of t = M_p & M_q stay in registers; the rest
+        // are re-read from the rows when needed
+        constexpr int TC = 2;
+        unsigned tc[TC];
         unsigned c4 = 0;
-        for (int u = 0; u < wq; u++) c4 += __popc(rows[pw + u] & rows[qw + u]);
+#pragma unroll
+        for (int u = 0; u < TC; u++) {
+            tc[u] = u < wq ? (rows[pw + u] & rows[qw + u]) : 0u;
+            c4 += __popc(tc[u]);
+        }
+        for (int u = TC; u < wq; u++) c4 += __popc(rows[pw + u] & rows[qw + u]);
         acc[4] += c4;
         if (a.kmax < 5) continue;
         for (int rw = 0; rw < wq; rw++) {
-            unsigned br = rows[pw + rw] & rows[qw + rw];
+            unsigned br = rw < TC ? pick(tc, rw) : (rows[pw + rw] & rows[qw + rw]);
             while (br) {
                 const int r = rw * 32 + __ffs(br) - 1;
                 br &= br - 1;
@@ -844,7 +862,10 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
                 const long long rr = (long long)r * w;
                 if (a.account) bytes += 8ull * wr;
                 unsigned c5 = 0;
-                for (int u = 0; u < wr; u++) c5 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
+#pragma unroll
+                for (int u = 0; u < TC; u++)
+                    if (u < wr) c5 += __popc(tc[u] & rows[rr + u]);
+                for (int u = TC; u < wr; u++) c5 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
                 acc[5] += c5;
                 if (a.kmax < 6) continue;
                 unsigned c6 = 0;
