@@ -219,6 +219,25 @@ __device__ __forceinline__ int list_lim(const KArgs &a, long long beg, int len) 
     return len > 0 ? a.adj[beg + len - 1] : -1;
 }
 
+// Number of distinct values among ex[0,nex) that lie in the first len
+// entries of N(v) (list at beg): the excluded chosen vertices a COUNT
+// without a father set must subtract from len. One membership test each
+// (bitmap word for hubs) instead of a sweep of the list.
+__device__ __forceinline__ int excluded_in_list(const KArgs &a, int v, long long beg, int len, const int *ex,
+                                                int nex) {
+    if (len <= 0) return 0;
+    const int h = hub_of(a, v);
+    const int lim = h >= 0 ? list_lim(a, beg, len) : 0;
+    int k = 0;
+    for (int q = 0; q < nex; q++) {
+        bool seen = false;
+        for (int q2 = 0; q2 < q; q2++) seen |= ex[q2] == ex[q];
+        if (seen) continue;
+        if (h >= 0 ? hub_has(a, h, ex[q], lim) : has(a.adj + beg, len, ex[q], a.lin_thr)) k++;
+    }
+    return k;
+}
+
 // Wave-cooperative intersection of stream[0,slen) with the set tested by
 // probe(x), skipping the (at most 8) excluded values; writes the sorted
 // result to out when MAT.
@@ -331,7 +350,9 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
         return o.kind == GATP_LIST ? wave_isect<true>(stream, slen, probe, out, ex, nex, C.ln)
                                    : wave_isect<false>(stream, slen, probe, out, ex, nex, C.ln);
     };
-    if (!has_r) {
+    if (!has_r && o.kind == GATP_COUNT) {
+        result = alen - uni(excluded_in_list(a, c, ab, alen, ex, nex));
+    } else if (!has_r) {
         result = run(A, alen, [](int) { return true; });
     } else if (sa) {
         // stream A, probe the father: LDS buffer, hub bitmap, or HBM list
@@ -461,6 +482,13 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
             int nex = 0;
             for (int f = 0; f < d; f++)
                 if (o.excl >> f & 1u) ex[nex++] = S->val[f];
+            if (o.father < 0) {
+                // |N(c) ∩ [0,t)| minus the excluded chosen vertices it holds
+                const bool live = pass && !dup;
+                S->leafcnt[o.leaf_idx][ln] = live ? alen - excluded_in_list(a, c, ab, alen, ex, nex) : 0;
+                if (a.account && live) bytes += 4ull * alen + 16ull;
+                continue;
+            }
             const int f = o.father;
             const bool has_r = f >= 0;
             bool b_lds = false;
