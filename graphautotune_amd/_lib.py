@@ -117,11 +117,12 @@ class PlanOptions(C.Structure):
     _fields_ = [
         ("schedule_policy", C.c_int32),
         ("schedule_rank", C.c_int32),
-        ("reserved", C.c_int64 * 3),
+        ("schedule_ranks", C.POINTER(C.c_int32)),
+        ("reserved", C.c_int64 * 2),
     ]
 
 
-SCHEDULE_REFERENCE, SCHEDULE_GRAPH = 0, 1
+SCHEDULE_REFERENCE, SCHEDULE_GRAPH, SCHEDULE_RANKED = 0, 1, 2
 
 
 class GatError(RuntimeError):

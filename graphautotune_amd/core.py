@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (MODE_LABELED, MODE_REFERENCE, MODE_UNIQUE, SCHEDULE_GRAPH, SCHEDULE_REFERENCE,
+from ._lib import (MODE_LABELED, MODE_REFERENCE, MODE_UNIQUE, SCHEDULE_GRAPH, SCHEDULE_RANKED, SCHEDULE_REFERENCE,
                    CountStats, KernelConfig, PlanOptions, ScheduleData, check, lib)
 
 CPU_DEVICE, GPU_DEVICE, UNKNOWN_DEVICE = 0, 1, 2  # consts/general_consts.hpp:6-10
@@ -223,7 +223,7 @@ class GlobalMemoryGraph:
 Graph = GlobalMemoryGraph
 
 
-SCHEDULES = {"reference": SCHEDULE_REFERENCE, "graph": SCHEDULE_GRAPH}
+SCHEDULES = {"reference": SCHEDULE_REFERENCE, "graph": SCHEDULE_GRAPH, "ranked": SCHEDULE_RANKED}
 
 
 class Plan:
@@ -232,7 +232,9 @@ class Plan:
     schedule="reference" uses the reference's Schedule for every pattern;
     schedule="graph" (unique/labeled modes) ranks the Schedule candidates by
     the reference's cost estimate under this graph's statistics and takes
-    the `schedule_rank`-th."""
+    the `schedule_rank`-th (an int for every pattern, or a list with one
+    rank per pattern); schedule="ranked" ranks them under the reference's
+    fixed statistics, so a rank names the same Schedule on every graph."""
 
     def __init__(self, graph, patterns, mode="unique", schedule="reference", schedule_rank=0):
         if isinstance(patterns, (str, Pattern)):
@@ -243,7 +245,13 @@ class Plan:
         arr = (C.c_char_p * len(self.patterns))(*[p.encode() for p in self.patterns])
         opt = PlanOptions()
         opt.schedule_policy = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
-        opt.schedule_rank = int(schedule_rank)
+        if isinstance(schedule_rank, (list, tuple)):
+            if len(schedule_rank) != len(self.patterns):
+                raise ValueError("%d schedule ranks for %d patterns" % (len(schedule_rank), len(self.patterns)))
+            self._ranks = (C.c_int32 * len(schedule_rank))(*[int(r) for r in schedule_rank])
+            opt.schedule_ranks = self._ranks
+        else:
+            opt.schedule_rank = int(schedule_rank)
         h = C.c_void_p()
         check(lib().gat_plan_create_ex(graph.handle, arr, len(self.patterns), self.mode, C.byref(opt),
                                        C.byref(h)))

@@ -218,6 +218,7 @@ int gat_plan_create_ex(gat_graph *g, const char *const *adj_strs, int32_t n_patt
     if (opt) {
         po.schedule_policy = opt->schedule_policy;
         po.schedule_rank = opt->schedule_rank;
+        if (opt->schedule_ranks) po.schedule_ranks.assign(opt->schedule_ranks, opt->schedule_ranks + n_patterns);
         if (po.schedule_policy == GAT_SCHEDULE_GRAPH && mode != GAT_MODE_REFERENCE) {
             std::lock_guard<std::mutex> lk(g->mu);
             const int64_t tri = graph_triangles(g);

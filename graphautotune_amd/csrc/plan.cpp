@@ -246,11 +246,16 @@ int compile_plan(const std::vector<std::string> &adj, int mode, const PlanOption
     if (adj.empty()) GAT_FAIL(GAT_EINVAL, "no patterns");
     if (mode != GAT_MODE_UNIQUE && mode != GAT_MODE_REFERENCE && mode != GAT_MODE_LABELED)
         GAT_FAIL(GAT_EINVAL, "unknown count mode %d", mode);
-    if (opt.schedule_policy != GAT_SCHEDULE_REFERENCE && opt.schedule_policy != GAT_SCHEDULE_GRAPH)
+    if (opt.schedule_policy != GAT_SCHEDULE_REFERENCE && opt.schedule_policy != GAT_SCHEDULE_GRAPH &&
+        opt.schedule_policy != GAT_SCHEDULE_RANKED)
         GAT_FAIL(GAT_EINVAL, "unknown schedule policy %d", opt.schedule_policy);
-    if (opt.schedule_policy == GAT_SCHEDULE_GRAPH && mode == GAT_MODE_REFERENCE)
+    if (opt.schedule_policy != GAT_SCHEDULE_REFERENCE && mode == GAT_MODE_REFERENCE)
         GAT_FAIL(GAT_EINVAL, "reference-mode counts are defined only for the reference's schedule");
     if (opt.schedule_rank < 0) GAT_FAIL(GAT_EINVAL, "negative schedule rank");
+    if (!opt.schedule_ranks.empty() && opt.schedule_ranks.size() != adj.size())
+        GAT_FAIL(GAT_EINVAL, "%zu schedule ranks for %zu patterns", opt.schedule_ranks.size(), adj.size());
+    for (int r : opt.schedule_ranks)
+        if (r < 0) GAT_FAIL(GAT_EINVAL, "negative schedule rank");
     if ((int)adj.size() > GATP_MAX_TERMS) GAT_FAIL(GAT_ELIMIT, "at most %d patterns per plan", GATP_MAX_TERMS);
     CompiledPlan cp;
     cp.relabel = mode != GAT_MODE_REFERENCE;
@@ -268,8 +273,10 @@ int compile_plan(const std::vector<std::string> &adj, int mode, const PlanOption
             std::vector<ScheduleCandidate> cands;
             rc = schedule_candidates(p, opt.stats, &cands);
             if (!rc)
-                rc = build_schedule_from(p, cands[std::min<size_t>((size_t)opt.schedule_rank, cands.size() - 1)],
-                                         &info.sched);
+            {
+                const int rank = opt.schedule_ranks.empty() ? opt.schedule_rank : opt.schedule_ranks[pi];
+                rc = build_schedule_from(p, cands[std::min<size_t>((size_t)rank, cands.size() - 1)], &info.sched);
+            }
         }
         if (rc) return rc;
         const Schedule &s = info.sched;

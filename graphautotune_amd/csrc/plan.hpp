@@ -38,12 +38,14 @@ struct CompiledPlan {
 uint64_t unique_restrictions(const Schedule &s, std::vector<Pair> *used);
 
 // How each pattern's Schedule is chosen. REFERENCE: the reference's pick
-// (the only choice whose reference-mode count is defined). GRAPH (unique and
-// labeled modes): the candidates ranked by the cost estimate under `stats`,
-// taking entry `rank` (clamped to the last one).
+// (the only choice whose reference-mode count is defined). GRAPH and RANKED
+// (unique and labeled modes): the candidates ranked by the cost estimate
+// under `stats` (the graph's own for GRAPH, the reference's fixed numbers for
+// RANKED), taking entry `rank` (clamped to the last one).
 struct PlanOptions {
     int schedule_policy = GAT_SCHEDULE_REFERENCE;
     int schedule_rank = 0;
+    std::vector<int> schedule_ranks;  // per pattern; empty = schedule_rank for all
     ScheduleStats stats;
 };
 

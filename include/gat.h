@@ -64,6 +64,10 @@ extern "C" {
 #define GAT_SCHEDULE_GRAPH 1     /* unique/labeled only: candidates ranked by the
                                     same estimate under this graph's vertex,
                                     directed-edge and triangle counts */
+#define GAT_SCHEDULE_RANKED 2    /* unique/labeled only: candidates ranked under
+                                    the reference's fixed statistics, so a rank
+                                    names the same Schedule on every graph
+                                    (rank 0 = the reference's pick) */
 
 /* ---- opaque handles -------------------------------------------------- */
 typedef struct gat_graph gat_graph; /* host CSR (+ device copy once used) */
@@ -158,7 +162,9 @@ typedef struct gat_plan_options {
     int32_t schedule_policy;  /* GAT_SCHEDULE_*                              */
     int32_t schedule_rank;    /* GRAPH: take the rank-th candidate (0 = best),
                                  clamped to the last one                      */
-    int64_t reserved[3];
+    const int32_t *schedule_ranks; /* GRAPH: one rank per pattern (overrides
+                                 schedule_rank), or NULL                      */
+    int64_t reserved[2];
 } gat_plan_options;
 
 /* ---- errors ---------------------------------------------------------- */

@@ -33,3 +33,20 @@ def test_batched_graph_schedules(gat, orc, gpu):
     want = [orc.unique_count(off, adj, p) for p in pats]
     for r in (0, 1, 3):
         assert gat.Plan(g, pats, "unique", schedule="graph", schedule_rank=r).count()[0] == want, r
+
+
+def test_ranked_schedules_and_tuner(gat, orc, gpu):
+    """The schedule tuner times ranked candidates on a proxy graph and keeps
+    the fastest per pattern; every candidate must count the oracle's value,
+    and the chosen ranks must reproduce it in a batched plan."""
+    from graphautotune_amd.tuning.schedules import tune_schedule_ranks
+    g = gat.Graph.chung_lu(200, 8.0, 2.1, 30, 5)
+    off, adj = g.csr()
+    pats = [HOUSE, C5, connected_patterns(6)[40]]
+    want = [orc.unique_count(off, adj, p) for p in pats]
+    ranks, hist = tune_schedule_ranks(g, pats, max_rank=6)
+    assert len(ranks) == len(pats) and all(0 <= r < 6 for r in ranks)
+    for h in hist:
+        assert h[0][0] == 0  # the reference's pick is always timed
+    got, _ = gat.Plan(g, pats, "unique", schedule="ranked", schedule_rank=ranks).count()
+    assert got == want

@@ -190,3 +190,28 @@ def test_graph_schedule_policy_rules(gat):
     last = gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=total - 1).schedule(0)
     past = gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=total + 5).schedule(0)
     assert bytes(last.data) == bytes(past.data)
+
+
+def test_per_pattern_schedule_ranks(gat):
+    g = gat.Graph.erdos_renyi(50, 0.2, 1)
+    stats = (g.v_cnt(), 2 * g.undirected_edges(), g.triangles())
+    pats = [HOUSE, C5]
+    plan = gat.Plan(g, pats, "unique", schedule="graph", schedule_rank=[3, 7])
+    for p, r, i in zip(pats, (3, 7), range(2)):
+        cands, total = gat.schedule_candidates(p, *stats, limit=64)
+        want = cands[min(r, total - 1)]
+        assert list(plan.schedule(i).data.perm) == list(want.data.perm)
+    with pytest.raises(ValueError):
+        gat.Plan(g, [HOUSE], "unique", schedule="graph", schedule_rank=[1, 2])
+
+
+def test_ranked_policy_is_graph_independent(gat):
+    """A "ranked" rank names the same Schedule on every graph: the candidates
+    ranked under the reference's fixed statistics; rank 0 is its pick."""
+    cands, total = gat.schedule_candidates(C5, limit=64)
+    graphs = [gat.Graph.erdos_renyi(50, 0.2, 1), gat.Graph.rmat(8, 8, 0.57, 0.19, 0.19, 1)]
+    for r in (0, 5, total - 1, total + 3):
+        for g in graphs:
+            s = gat.Plan(g, [C5], "unique", schedule="ranked", schedule_rank=r).schedule(0)
+            assert bytes(s.data) == bytes(cands[min(r, total - 1)].data)
+    assert bytes(cands[0].data) == bytes(gat.Schedule(C5).data)
