@@ -76,7 +76,8 @@ struct WaveState {
     int f_cur[GATP_MAX_DEPTH + 1];
     int f_hi[GATP_MAX_DEPTH + 1];
     int leafcnt[GATP_MAX_LEAF_OPS][W];
-    unsigned long long bytes;
+    unsigned long long bytes;   // committed W_alg of this wave
+    unsigned long long ebytes;  // W_alg of the work not yet committed
     int abort;
     int pad;
 };
@@ -385,7 +386,7 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
         S->slen[slot] = result;
         if (a.account) {
             const GatSlot *fs = has_r ? &C.P.slots[f] : nullptr;
-            S->bytes += 4ull * (alen + blen) + (o.kind == GATP_LIST ? 4ull * result : 0ull) + 16ull +
+            S->ebytes += 4ull * (alen + blen) + (o.kind == GATP_LIST ? 4ull * result : 0ull) + 16ull +
                         ((fs && fs->kind == GATP_ALIAS) ? 16ull : 0ull);
         }
     }
@@ -569,7 +570,7 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
     }
     if (a.account) {
         bytes = wave_sum64(bytes);
-        if (ln == 0) S->bytes += bytes;
+        if (ln == 0) S->ebytes += bytes;
     }
 }
 
@@ -592,12 +593,40 @@ __device__ __forceinline__ void push_frame(const Ctx &C, int d, int node, int lo
     wsync();
 }
 
+// Pending counts and W_alg of the edges finished since the last commit go
+// into the wave's totals; an abandoned edge's are dropped instead.
+__device__ __forceinline__ void commit_pending(WaveState *S, int np, int ln) {
+    for (int i = ln; i < np; i += W) {
+        S->wave_acc[i] += S->task_acc[i];
+        S->task_acc[i] = 0;
+    }
+    if (ln == 0) {
+        S->bytes += S->ebytes;
+        S->ebytes = 0;
+    }
+    wsync();
+}
+__device__ __forceinline__ void drop_pending(WaveState *S, int np, int ln) {
+    for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
+    if (ln == 0) {
+        S->ebytes = 0;
+        S->abort = 0;
+    }
+    wsync();
+}
+
+// Depth-first walk of the trie for the directed edges [e0, e1). When the
+// root has one child (a single depth-1 node, so each edge is one subtree),
+// an edge whose sets outgrow the LDS buffers is handed alone to the HBM
+// launch and the walk goes on with the next edge; otherwise the whole task
+// is abandoned and re-run there.
 template <bool GS>
 __device__ void run_task(const Ctx &C, long long e0, long long e1) {
     const KArgs &a = C.a;
     const GatPlanDev &P = C.P;
     WaveState *S = C.S;
     const GatNode &root = P.nodes[0];
+    const bool edge_grain = !GS && root.child_count == 1;
     long long e = e0;
     while (e < e1) {
         // row of edge e: last v with off[v] <= e
@@ -630,7 +659,18 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
                     if (C.ln == 0) S->f_cur[d] = cur + 1;
                     wsync();
                     if (N.excl && equals_any(S, N.excl, d, c)) continue;
-                    if (!visit<GS>(C, N, d, c)) return;  // abandoned (buffer too small)
+                    if (edge_grain && d == 1) commit_pending(S, P.n_patterns, C.ln);  // previous edge done
+                    if (!visit<GS>(C, N, d, c)) {  // a set outgrew its LDS buffer
+                        if (!edge_grain) return;  // the whole task goes to the HBM launch
+                        if (C.ln == 0) {
+                            const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
+                            const long long edge = rb + (uni(S->f_cur[1]) - 1);
+                            if ((long long)idx < a.ovf_cap) a.ovf[idx] = edge; else atomicOr(a.err, 2u);
+                        }
+                        drop_pending(S, P.n_patterns, C.ln);
+                        d = 1;
+                        continue;
+                    }
                     if (N.child_count > 0) {
                         d++;
                         push_frame<GS>(C, d, N.child_begin, 0, INT_MAX);
@@ -648,6 +688,7 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
         }
         e = se;
     }
+    if (edge_grain) commit_pending(S, P.n_patterns, C.ln);
 }
 
 template <bool GS>
@@ -665,7 +706,10 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
     Ctx C{a, P, S, store, ln};
     const int np = P.n_patterns;
     for (int i = ln; i < np; i += W) S->wave_acc[i] = 0;
-    if (ln == 0) S->bytes = 0;
+    if (ln == 0) {
+        S->bytes = 0;
+        S->ebytes = 0;
+    }
     wsync();
     for (;;) {
         long long e0, e1;
@@ -695,8 +739,13 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
                 const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
                 if ((long long)idx < a.ovf_cap) a.ovf[idx] = e0; else atomicOr(a.err, 2u);
             }
+            if (ln == 0) S->ebytes = 0;  // the HBM launch does this task again
         } else {
             for (int i = ln; i < np; i += W) S->wave_acc[i] += S->task_acc[i];
+            if (ln == 0) {
+                S->bytes += S->ebytes;
+                S->ebytes = 0;
+            }
         }
         wsync();
     }
@@ -1369,7 +1418,9 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     }
     if (cfg.vertex_store == 2 && !clique)
         GAT_FAIL(GAT_EINFEASIBLE, "bitmap vertex store needs a plan of cliques K3..K6 in unique or labeled mode");
-    const long long ovf_cap = clique ? 1 : (g.m + chunk - 1) / chunk + 1;
+    // single-child roots hand over single edges (run_task), others whole tasks
+    const bool edge_grain = pd.nodes[0].child_count == 1;
+    const long long ovf_cap = clique ? 1 : (edge_grain ? g.m + 1 : (g.m + chunk - 1) / chunk + 1);
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
         for (auto &t : d->tier)
@@ -1423,6 +1474,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     L->wave_bytes = wave_bytes;
     L->ao = a;
     L->ao.cap = std::max(1, g.max_deg);
+    if (edge_grain) L->ao.chunk = 1;
     L->ovf_grid = dim3(d->ovf_grid);
     L->ovf_wave_bytes = int((sizeof(gatk::WaveState) + 15) / 16 * 16);
     L->ovf_lds = L->ovf_wave_bytes;

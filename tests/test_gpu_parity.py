@@ -213,3 +213,18 @@ def test_bitmap_overflow_and_labeled(gat, orc, gpu):
     assert got == want and st["overflow_tasks"] > 0
     lab, _ = gat.Plan(g, [clique(4), clique(5)], "labeled").count({"vertex_store": 2})
     assert lab == [24 * want[0], 120 * want[1]]
+
+
+def test_walg_and_counts_exact_under_overflow(gat, gpu):
+    """A 16-entry LDS set capacity pushes work to the HBM launch (single
+    edges when the root has one child, whole tasks otherwise); the counts and
+    the W_alg of the step must equal those of a run without overflow."""
+    g = gat.Graph.chung_lu(500, 10.0, 2.1, 80, 9)
+    # C5 materialises no set (its plan has no LIST op), so nothing overflows
+    for pats, overflows in (([HOUSE], True), ([C5], False), (connected_patterns(5), True)):
+        plan = gat.Plan(g, pats, "unique")
+        c1, s1 = plan.count({"vertex_store": 1}, account=True)
+        c2, s2 = plan.count({"vertex_store": 1, "slot_capacity": 16}, account=True)
+        assert c1 == c2, pats
+        assert (s2["overflow_tasks"] > 0) == overflows, pats
+        assert s1["alg_bytes"] == s2["alg_bytes"], (pats, s1["alg_bytes"], s2["alg_bytes"])
