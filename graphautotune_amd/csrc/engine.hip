@@ -776,7 +776,15 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
 constexpr int CMAXW = 8;   // main tier: d <= 256
 constexpr int CMIDW = 16;  // workgroup tier 0: d <= 512, rows in LDS
 constexpr int CBIGW = 32;  // workgroup tier 1: d <= 1024, rows in LDS when it fits, else HBM
-constexpr int CSPLIT = 4;     // work items per vertex in tiers 1 and 2
+// Workgroup-tier items are (vertex, part) pairs, vertex * CSPLIT_ENC + part:
+// the parts of a vertex share out its pairs (each builds the rows).
+constexpr int CSPLIT_ENC = 16;
+constexpr int CSPLIT12 = 4;   // parts per vertex in tiers 1 and 2
+constexpr int CSPLIT0 = 1;    // parts per tier-0 vertex with d > CSPLIT0_D
+constexpr int CSPLIT0_D = 256;
+__device__ __forceinline__ int clique_parts(int tier, int d) {
+    return tier ? CSPLIT12 : (d > CSPLIT0_D ? CSPLIT0 : 1);
+}
 constexpr int CBLOCKW0 = 8;   // waves per workgroup, tier 0
 constexpr int CBLOCKW1 = 16;  // waves per workgroup, tiers 1 and 2 (few, heavy vertices)
 
@@ -995,14 +1003,13 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             if (d < 2) continue;
             if (d > a.cap) {
                 if (ln == 0) {
-                    // heavy vertices (tiers 1, 2) are split into CSPLIT work
-                    // items that share out the pairs
                     const int t = d <= 32 * CMIDW ? 0 : (d <= 32 * CBIGW ? 1 : 2);
-                    const int parts = t ? CSPLIT : 1;
+                    const int parts = clique_parts(t, d);
+                    atomicAdd(&a.ovf_n[3], 1ull);  // vertices handed on (statistics)
                     const unsigned long long idx = atomicAdd(&a.ovf_n[t], (unsigned long long)parts);
                     if ((long long)(idx + parts) <= a.ovf_cap) {
                         // tier 0 items are vertex ids; later tiers (vertex, part) pairs
-                        for (int k = 0; k < parts; k++) a.tier[t][idx + k] = t ? (long long)v0 * CSPLIT + k : v0;
+                        for (int k = 0; k < parts; k++) a.tier[t][idx + k] = (long long)v0 * CSPLIT_ENC + k;
                     } else {
                         atomicOr(a.err, 4u);
                     }
@@ -1069,10 +1076,10 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
         __syncthreads();
         if (vsh < 0) break;
         // tier 0 items are vertices; later tiers carry (vertex, part)
-        const int v0 = (int)(TIER ? vsh / CSPLIT : vsh);
-        const int part = (int)(TIER ? vsh % CSPLIT : 0);
-        const int parts = TIER ? CSPLIT : 1;
+        const int v0 = (int)(vsh / CSPLIT_ENC);
+        const int part = (int)(vsh % CSPLIT_ENC);
         const int d = a.split[v0];
+        const int parts = clique_parts(TIER, d);
         const long long rb = a.off[v0];
         const int w = (d + 31) >> 5;
         if (d > (MW > 0 ? lcap : a.hcap) || d > a.hcap) {  // cannot happen: lists are split by d
@@ -1431,7 +1438,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         d->ovf_cap = ovf_cap;
     }
     if (clique && !d->tier[0])
-        for (auto &t : d->tier) HIPCHECK(hipMalloc(&t, sizeof(long long) * (size_t)(g.n + 1) * gatk::CSPLIT));
+        for (auto &t : d->tier) HIPCHECK(hipMalloc(&t, sizeof(long long) * (size_t)(g.n + 1) * gatk::CSPLIT_ENC));
     const int wave_bytes = int((sizeof(gatk::WaveState) + size_t(pd.n_store) * cfg.slot_capacity * 4 + 15) / 16 * 16);
     const size_t lds = size_t(wave_bytes) * cfg.block_waves;
     if (lds > di.lds_per_block)
@@ -1492,7 +1499,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         c.queue = a.queue;
         for (int t = 0; t < 3; t++) c.tier[t] = d->tier[t];
         c.ovf_n = a.ovf_n;
-        c.ovf_cap = (long long)(g.n + 1) * gatk::CSPLIT;
+        c.ovf_cap = (long long)(g.n + 1) * gatk::CSPLIT_ENC;
         c.bytes = a.bytes;
         c.err = a.err;
         c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
@@ -1587,7 +1594,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
 static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf[3] = {0, 0, 0}, bytes[3] = {0, 0, 0};
+    unsigned long long raw[GATP_MAX_TERMS], ovf[4] = {0, 0, 0, 0}, bytes[3] = {0, 0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
@@ -1613,7 +1620,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->main_kernel_ms = ms_main;
             st->second_kernel_ms = ms_second;
         }
-        st->overflow_tasks = ovf[0] + (ovf[1] + ovf[2]) / gatk::CSPLIT;  // clique tiers 1, 2: CSPLIT parts each
+        st->overflow_tasks = L.clique ? ovf[3] : ovf[0];
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
         st->alg_bytes_second = bytes[1];

@@ -123,3 +123,135 @@ def rank_study(model, scale, patterns, pool_size, budget, seed=0, log=print, max
         log(json.dumps(res))
         out.append(res)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Schedule study: the same cost model ranking Schedule candidates instead of
+# kernel configs. Rank r of the "ranked" policy names one Schedule of a
+# pattern on every graph (candidates ordered by the reference's estimate
+# under its fixed statistics); rank 0 is the reference's own pick, so the
+# method without the learned part is "always rank 0".
+
+def schedule_featurizer(graph, pattern, rank):
+    plan = core.Plan(graph, [pattern], "unique", schedule="ranked", schedule_rank=rank)
+    pf = features.pattern_features(plan.schedule(0), 1)
+    gf = features.graph_features(graph)
+    return features.vector(pf, gf, Config())
+
+
+def _time_rank(graph, pattern, rank):
+    plan = core.Plan(graph, [pattern], "unique", schedule="ranked", schedule_rank=rank)
+    plan.count()
+    counts, st = plan.count()
+    return counts[0], st["seconds"]
+
+
+def collect_schedules(scales, patterns, max_rank, log=print, max_default_s=0.3):
+    """Time every candidate rank < max_rank of each pattern on R-MAT graphs
+    (default kernel config); a pattern whose rank 0 takes longer than
+    max_default_s at a scale is skipped there."""
+    X, y, meta = [], [], []
+    for sc in scales:
+        g = core.Graph.rmat(sc, 16, 0.57, 0.19, 0.19, sc)
+        for p in patterns:
+            _, total = core.schedule_candidates(p, limit=1)
+            c0, t0 = _time_rank(g, p, 0)
+            if t0 > max_default_s:
+                log("collect s%d %s: rank 0 %.1f ms, skipped" % (sc, p, t0 * 1e3))
+                continue
+            ts = [t0]
+            for r in range(1, min(max_rank, total)):
+                c, t = _time_rank(g, p, r)
+                if c != c0:
+                    raise AssertionError("rank %d of %s counts %d, rank 0 counts %d" % (r, p, c, c0))
+                ts.append(t)
+            for r, t in enumerate(ts):
+                X.append(schedule_featurizer(g, p, r))
+                y.append(t)
+                meta.append({"scale": sc, "pattern": p, "rank": r, "seconds": t})
+            log("collect s%d %s: %d ranks, %.2f .. %.2f ms" % (sc, p, len(ts), min(ts) * 1e3, max(ts) * 1e3))
+    return np.stack(X), np.asarray(y), meta
+
+
+def schedule_rank_study(model, scale, patterns, max_rank, budget, seed=0, log=print):
+    """For each pattern: time every candidate rank (the grid judge), let the
+    model rank them, re-measure nothing beyond its top `budget` picks, and
+    compare with rank 0 (the reference's estimate) and with `budget` random
+    picks."""
+    rng = random.Random(seed + 2)
+    g = core.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, scale)
+    out = []
+    for p in patterns:
+        _, total = core.schedule_candidates(p, limit=1)
+        n = min(max_rank, total)
+        truth = []
+        for r in range(n):
+            truth.append(_time_rank(g, p, r)[1])
+            log("  rank %d: %.2f ms" % (r, truth[-1] * 1e3))
+        truth = np.array(truth)
+        pred = model.predict(np.stack([schedule_featurizer(g, p, r) for r in range(n)]))
+        k = min(budget, n)
+        picks = list(np.argsort(pred)[:k])
+        best = truth.min()
+        rnd = [truth[rng.sample(range(n), k)].min() for _ in range(200)]
+        res = {
+            "pattern": p, "scale": scale, "candidates": n, "budget": k,
+            "grid_best_ms": best * 1e3,
+            "reference_rank0_ms": truth[0] * 1e3,
+            "model_pick_ms": truth[picks].min() * 1e3,
+            "model_picks": [int(i) for i in picks],
+            "model_regret": truth[picks].min() / best - 1.0,
+            "reference_regret": truth[0] / best - 1.0,
+            "random_regret_mean": float(np.mean(rnd)) / best - 1.0,
+            "spearman": spearman(pred, truth) if n > 2 else None,
+        }
+        log(json.dumps(res))
+        out.append(res)
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import os
+    import sys
+    ap = argparse.ArgumentParser(description="learned schedule-ranking study")
+    ap.add_argument("--train-scales", default="9,10")
+    ap.add_argument("--eval-scale", type=int, default=11)
+    ap.add_argument("--max-rank", type=int, default=12)
+    ap.add_argument("--budget", type=int, default=2)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    from .batched_study import connected_patterns
+    p6 = connected_patterns(6)
+    rng = np.random.default_rng(6)
+    held = [p6[i] for i in rng.choice(len(p6), 10, replace=False)]  # the batched set's 6-vertex patterns
+    train6 = [p for p in p6 if p not in held]
+    train = connected_patterns(5) + [train6[i] for i in np.random.default_rng(7).choice(len(train6), 24,
+                                                                                          replace=False)]
+    t0 = time.time()
+    X, y, meta = collect_schedules([int(s) for s in args.train_scales.split(",")], train, args.max_rank, log=log)
+    log("collected %d timings in %.1f s" % (len(y), time.time() - t0))
+    model = MLPCostModel(features.FEATURES, 256, seed=0)
+    loss = model.fit(X, y, steps=800, lr=1e-3)
+    log("trained: loss %.4f" % loss)
+    res = schedule_rank_study(model, args.eval_scale, held, args.max_rank, args.budget, log=log)
+    summary = {
+        "train": {"scales": args.train_scales, "patterns": len(train), "timings": len(y), "loss": loss},
+        "eval": {"scale": args.eval_scale, "patterns": len(held), "budget": args.budget},
+        "mean_model_regret": float(np.mean([r["model_regret"] for r in res])),
+        "mean_reference_regret": float(np.mean([r["reference_regret"] for r in res])),
+        "mean_random_regret": float(np.mean([r["random_regret_mean"] for r in res])),
+        "sum_grid_best_ms": float(sum(r["grid_best_ms"] for r in res)),
+        "sum_model_pick_ms": float(sum(r["model_pick_ms"] for r in res)),
+        "sum_reference_ms": float(sum(r["reference_rank0_ms"] for r in res)),
+        "results": res,
+    }
+    print(json.dumps({k: v for k, v in summary.items() if k != "results"}), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(summary, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
