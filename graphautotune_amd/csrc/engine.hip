@@ -814,15 +814,16 @@ struct CArgs {
 };
 
 // Rows M_p of the list L (d entries) into rows (w words per row, already
-// zeroed), for the row blocks first, first + step, ... of 64 rows: lanes
-// sweep the (row, element) pairs of 64 rows at a time; each element of
-// N+(L[p]) is searched in L[0,p), or each L[q], q < p, in N+(L[p]) when that
-// list is gallop_ratio times longer than p.
+// zeroed), for the rows p = first, first + stride, first + 2 stride, ...
+// (a workgroup's waves interleave their rows, so light and heavy rows mix):
+// lanes sweep the (row, element) pairs of 64 of these rows at a time; each
+// element of N+(L[p]) is searched in L[0,p), or each L[q], q < p, in
+// N+(L[p]) when that list is gallop_ratio times longer than p.
 template <class LP, class RP>
-__device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first, int step, int ln,
+__device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first, int stride, int ln,
                          unsigned long long &bytes) {
-    for (int cb = first; cb < d; cb += step) {
-        const int ci = cb + ln;
+    for (int cb = 0; first + cb * stride < d; cb += W) {
+        const int ci = first + (cb + ln) * stride;
         const bool valid = ci < d;
         const int x = valid ? L[ci] : 0;
         const long long xb = valid ? a.off[x] : 0;
@@ -845,7 +846,7 @@ __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first,
             const long long s_xb = bperm64(xb, so);
             const int s_xl = bperm(xl, so);
             if (j < T) {
-                const int s_ci = cb + so;
+                const int s_ci = first + (cb + so) * stride;
                 const int idx = j - s_off;
                 int pos = -1;
                 if (s_sx) {
@@ -1022,7 +1023,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             for (int k = ln; k < d * w; k += W) rows[k] = 0u;
             if (a.account && ln == 0) bytes += 4ull * d + 16ull;
             wsync();
-            vtx_rows(a, L, d, rows, w, 0, W, ln, bytes);
+            vtx_rows(a, L, d, rows, w, 0, 1, ln, bytes);
             wsync();
             // L is no longer needed: it holds the pair offsets from here on
             int *off = L;
@@ -1095,7 +1096,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             if (!LDS) __threadfence();
             __syncthreads();
             const int *L = LDS ? Ls : a.adj + rb;
-            vtx_rows(a, L, d, rows, w, wid * W, W * NW, ln, build_bytes);
+            vtx_rows(a, L, d, rows, w, wid, NW, ln, build_bytes);
             if (!LDS) __threadfence();
             __syncthreads();
             // pair offsets: block-wide exclusive prefix of popc(M_p)
