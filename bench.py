@@ -8,7 +8,9 @@ resident in HBM). The plain single-pattern engine (default config) is timed
 beside it; `batched_vs_plain` = (t_K4 + t_K5) / t_batched at identical
 counts. `roofline` is for whichever of the main and the second launch takes
 longer: W_alg bytes of that launch (instrumented pass) / its hipEvent-timed
-duration, against 8 TB/s. `cpu_baseline` times the CPU oracle's restatement
+duration, against 8 TB/s; `hbm_achieved` is the same launch's measured HBM
+bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) over
+that duration. `cpu_baseline` times the CPU oracle's restatement
 of the same algorithm on a bounded sample of top vertices and scales it by
 algorithmic bytes.
 
@@ -229,6 +231,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
+            # measured HBM rate of the same launch: far below `achieved` when
+            # the operands are read from LDS (the bitmap rows of the
+            # workgroup tiers), so `frac` can exceed 1 there
+            "hbm_achieved": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
+            "hbm_frac": traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None,
             "alg_bytes_per_launch": kernel_walg,
             "kernel": kernel_name,
             "kernel_ms": kernel_ms,
