@@ -125,6 +125,19 @@ class PlanOptions(C.Structure):
 SCHEDULE_REFERENCE, SCHEDULE_GRAPH, SCHEDULE_RANKED = 0, 1, 2
 
 
+class CountInfo(C.Structure):
+    """gat_pattern_count_info: divisor, multiplier and restrictions of one
+    pattern of a plan."""
+    _fields_ = [
+        ("divisor", C.c_uint64),
+        ("multiplier", C.c_uint64),
+        ("relabeled", C.c_int32),
+        ("n_restrictions", C.c_int32),
+        ("restr_first", C.c_int32 * (MAXV * MAXV)),
+        ("restr_second", C.c_int32 * (MAXV * MAXV)),
+    ]
+
+
 class GatError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("gat error %d: %s" % (code, msg))
@@ -162,6 +175,9 @@ _SIGS = {
                             C.POINTER(CountStats)]),
     "gat_count_graph": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_int32, C.POINTER(C.c_uint64),
                                   C.POINTER(CountStats)]),
+    "gat_plan_count_info": (C.c_int, [_P, C.c_int32, C.POINTER(CountInfo), C.c_size_t]),
+    "gat_count_shard": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_uint32, C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "gat_plan_free": (None, [_P]),
     "gat_plan_describe": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "gat_default_config": (C.c_int, [C.POINTER(KernelConfig)]),

@@ -290,6 +290,31 @@ class Plan:
         stats = {k: getattr(st, k) for k, _ in CountStats._fields_ if k != "reserved"}
         return [int(x) for x in out], stats
 
+    def count_info(self, i=0):
+        """{"divisor", "multiplier", "relabeled", "restrictions": [(a, b), ...]}
+        of pattern i: the raw count is the number of injective, edge-preserving
+        maps psi of the Schedule-ordered pattern with psi(b) < psi(a) for every
+        restriction (unique and labeled modes), and count() reports
+        raw / divisor * multiplier."""
+        d = _lib.CountInfo()
+        check(lib().gat_plan_count_info(self._h, i, C.byref(d), C.sizeof(d)))
+        return {"divisor": int(d.divisor), "multiplier": int(d.multiplier), "relabeled": bool(d.relabeled),
+                "restrictions": [(d.restr_first[k], d.restr_second[k]) for k in range(d.n_restrictions)]}
+
+    def count_shard(self, root_begin, root_end, config=None, account=False):
+        """Raw sums (one per pattern) of the embeddings whose Schedule vertex 0
+        maps to a root in [root_begin, root_end) of the graph the plan runs on,
+        and a stats dict. Over a partition of the roots the raw sums add up to
+        divisor * count."""
+        n = len(self.patterns)
+        out = (C.c_uint64 * n)()
+        st = CountStats()
+        cfg = config if isinstance(config, KernelConfig) else KernelConfig.from_dict(config or {})
+        check(lib().gat_count_shard(self._h, C.byref(cfg), _lib.FLAG_ACCOUNT if account else 0, int(root_begin),
+                                    int(root_end), out, C.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in CountStats._fields_ if k != "reserved"}
+        return [int(x) for x in out], stats
+
 
 def count(graph, patterns, mode="unique", config=None, account=False):
     """Exact counts of every pattern in one batched launch."""

@@ -255,6 +255,24 @@ int gat_plan_schedule(const gat_plan *p, int32_t i, gat_schedule_data *out, size
     return GAT_OK;
 }
 
+int gat_plan_count_info(const gat_plan *p, int32_t i, gat_pattern_count_info *out, size_t cap) {
+    if (!p || !out || i < 0 || i >= (int)p->cp.patterns.size()) GAT_FAIL(GAT_EINVAL, "bad arguments");
+    if (cap < sizeof(gat_pattern_count_info)) GAT_FAIL(GAT_EINVAL, "count-info buffer too small");
+    std::memset(out, 0, sizeof(*out));
+    const PatternInfo &pi = p->cp.patterns[i];
+    out->divisor = pi.divisor;
+    out->multiplier = pi.multiplier;
+    out->relabeled = p->cp.relabel ? 1 : 0;
+    const auto &R = pi.used_restrictions;
+    if (R.size() > size_t(GAT_MAX_VERTEXES * GAT_MAX_VERTEXES)) GAT_FAIL(GAT_ELIMIT, "too many restrictions");
+    out->n_restrictions = (int32_t)R.size();
+    for (size_t k = 0; k < R.size(); k++) {
+        out->restr_first[k] = R[k].first;
+        out->restr_second[k] = R[k].second;
+    }
+    return GAT_OK;
+}
+
 void gat_plan_free(gat_plan *p) { delete p; }
 
 int gat_default_config(gat_kernel_config *out) {

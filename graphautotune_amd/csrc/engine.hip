@@ -40,6 +40,7 @@ struct KArgs {
     const int *split;  // split[v] = number of neighbours of v below v
     int n;
     long long m;
+    long long e_lo, e_hi;        // root edges of this count: [e_lo, e_hi)
     const GatPlanDev *plan;
     unsigned long long *counts;  // [n_patterns]
     unsigned long long *queue;   // [0] main queue head, [1] overflow head
@@ -723,10 +724,10 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
         } else {
             unsigned long long r = 0;
             if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
-            e0 = uni64((long long)__shfl(r, 0));
-            if (e0 >= a.m) break;
+            e0 = a.e_lo + uni64((long long)__shfl(r, 0));
+            if (e0 >= a.e_hi) break;
         }
-        e1 = e0 + a.chunk < a.m ? e0 + a.chunk : a.m;
+        e1 = e0 + a.chunk < a.e_hi ? e0 + a.chunk : a.e_hi;
         for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
         if (ln == 0) S->abort = 0;
         wsync();
@@ -794,6 +795,7 @@ struct CArgs {
     const int *split;
     int n;
     long long m;
+    int root_lo, root_hi;        // top vertices of this count: [root_lo, root_hi)
     unsigned long long *counts;
     int pat_of_k[8];
     int kmax;
@@ -995,9 +997,9 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     for (;;) {
         unsigned long long r = 0;
         if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
-        const long long v_lo = uni64((long long)__shfl(r, 0));
-        if (v_lo >= a.n) break;
-        const long long v_hi = v_lo + a.chunk < a.n ? v_lo + a.chunk : a.n;
+        const long long v_lo = a.root_lo + uni64((long long)__shfl(r, 0));
+        if (v_lo >= a.root_hi) break;
+        const long long v_hi = v_lo + a.chunk < a.root_hi ? v_lo + a.chunk : a.root_hi;
         for (long long vv = v_lo; vv < v_hi; vv++) {
             const int v0 = (int)vv;
             const int d = uni(a.split[v0]);
@@ -1009,7 +1011,6 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                     atomicAdd(&a.ovf_n[3], 1ull);  // vertices handed on (statistics)
                     const unsigned long long idx = atomicAdd(&a.ovf_n[t], (unsigned long long)parts);
                     if ((long long)(idx + parts) <= a.ovf_cap) {
-                        // tier 0 items are vertex ids; later tiers (vertex, part) pairs
                         for (int k = 0; k < parts; k++) a.tier[t][idx + k] = (long long)v0 * CSPLIT_ENC + k;
                     } else {
                         atomicOr(a.err, 4u);
@@ -1076,7 +1077,6 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
         }
         __syncthreads();
         if (vsh < 0) break;
-        // tier 0 items are vertices; later tiers carry (vertex, part)
         const int v0 = (int)(vsh / CSPLIT_ENC);
         const int part = (int)(vsh % CSPLIT_ENC);
         const int d = a.split[v0];
@@ -1372,7 +1372,19 @@ struct Launch {
     dim3 t1_grid;       // workgroup tier 1 (LDS when t1_lds > 0, else HBM)
     size_t t1_lds = 0;
     dim3 big_grid;      // workgroup tier 2 (HBM)
+    long long tasks = 0;  // root edges of the count
 };
+
+// Restrict a prepared count to the root vertices [lo, hi): the array store
+// takes the directed edges of those vertices, the bitmap store those top
+// vertices.
+static void set_roots(Launch *L, const Graph &g, int32_t lo, int32_t hi) {
+    L->a.e_lo = L->ao.e_lo = g.off[lo];
+    L->a.e_hi = L->ao.e_hi = g.off[hi];
+    L->ca.root_lo = lo;
+    L->ca.root_hi = hi;
+    L->tasks = g.off[hi] - g.off[lo];
+}
 
 static void fill_defaults(gat_kernel_config *c) {
     if (c->block_waves <= 0) c->block_waves = 1;
@@ -1451,6 +1463,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     a.split = d->dg->split;
     a.n = g.n;
     a.m = g.m;
+    a.e_lo = 0;
+    a.e_hi = g.m;
     a.plan = d->plan;
     a.counts = d->counts;
     a.queue = d->queue;
@@ -1494,6 +1508,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         c.split = a.split;
         c.n = a.n;
         c.m = a.m;
+        c.root_lo = 0;
+        c.root_hi = a.n;
         c.counts = a.counts;
         for (int k = 0; k < 8; k++) c.pat_of_k[k] = p->cp.clique_pattern[k];
         c.kmax = 31 - __builtin_clz(p->cp.clique_ks);
@@ -1546,6 +1562,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         L->t1_lds = t1 <= di.lds_per_block ? t1 : 0;
         L->t1_grid = L->t1_lds ? dim3(std::max(1, (int)(di.lds_per_cu / t1)) * di.cus) : L->big_grid;
     }
+    set_roots(L, g, 0, g.n);
     return GAT_OK;
 }
 
@@ -1592,7 +1609,10 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     return GAT_OK;
 }
 
-static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed) {
+// counts_out: the requested counts, or with raw set the engine's raw sums
+// (a shard's share of divisor * count, before the divisor and multiplier).
+static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_stats *st, bool timed,
+                  bool raw_sums = false) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
     unsigned long long raw[GATP_MAX_TERMS], ovf[4] = {0, 0, 0, 0}, bytes[3] = {0, 0, 0};
@@ -1603,7 +1623,9 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
     HIPCHECK(hipMemcpyAsync(&err, d->err, sizeof(err), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipStreamSynchronize(d->stream));
     if (err) GAT_FAIL(GAT_ENODEV, "engine reported internal error flags 0x%x", err);
-    for (int i = 0; i < np; i++) {
+    for (int i = 0; i < np && raw_sums; i++)
+        if (counts_out) counts_out[i] = raw[i];
+    for (int i = 0; i < np && !raw_sums; i++) {
         const PatternInfo &pi = p->cp.patterns[i];
         if (raw[i] % pi.divisor)
             GAT_FAIL(GAT_EINVAL, "internal: raw count %llu of pattern %d not divisible by %llu", raw[i], i,
@@ -1625,7 +1647,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
         st->alg_bytes_second = bytes[1];
-        st->tasks = (uint64_t)p->graph->m;
+        st->tasks = (uint64_t)L.tasks;
         st->grid_blocks = (int)L.grid.x;
         st->block_threads = (int)L.block.x;
         st->lds_bytes = (int)L.lds;
@@ -1653,6 +1675,21 @@ extern "C" int gat_count(gat_plan *p, const gat_kernel_config *cfg, uint32_t fla
     rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true);
     if (rc) return rc;
     return finish(p, L, counts_out, stats, true);
+}
+
+extern "C" int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags, int32_t root_begin,
+                               int32_t root_end, uint64_t *raw_out, gat_count_stats *stats) {
+    if (!p) GAT_FAIL(GAT_EINVAL, "null plan");
+    const int32_t n = p->graph->n;
+    if (root_begin < 0 || root_end < root_begin || root_end > n)
+        GAT_FAIL(GAT_EINVAL, "root range [%d, %d) is not within [0, %d)", root_begin, root_end, n);
+    Launch L;
+    int rc = prepare(p, cfg, &L);
+    if (rc) return rc;
+    set_roots(&L, *p->graph, root_begin, root_end);
+    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true);
+    if (rc) return rc;
+    return finish(p, L, raw_out, stats, true, true);
 }
 
 extern "C" int gat_count_graph(gat_plan *p, const gat_kernel_config *cfg, int32_t replays, uint64_t *counts_out,

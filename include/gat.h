@@ -148,7 +148,7 @@ typedef struct gat_count_stats {
     uint64_t alg_bytes;       /* W_alg of the whole count (GAT_FLAG_ACCOUNT)  */
     uint64_t alg_bytes_main;  /* W_alg of the main launch alone              */
     uint64_t alg_bytes_second;/* W_alg of the second launch alone            */
-    uint64_t tasks;           /* directed edges in the task space            */
+    uint64_t tasks;           /* directed edges of the roots counted         */
     int32_t grid_blocks;      /* launched workgroups                         */
     int32_t block_threads;
     int32_t lds_bytes;        /* dynamic LDS per workgroup                   */
@@ -252,6 +252,42 @@ int gat_count(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
  * `replays` times; counts and stats are those of the last replay. */
 int gat_count_graph(gat_plan *p, const gat_kernel_config *cfg, int32_t replays,
                     uint64_t *counts_out, gat_count_stats *stats);
+
+/* How pattern i of a plan is counted. In unique and labeled modes the raw
+ * count is the number of maps psi from the Schedule-ordered pattern
+ * (adj_rows of gat_plan_schedule) into the graph the plan runs on that are
+ * injective, send pattern edges to graph edges and satisfy psi(b) < psi(a)
+ * for every restriction (a, b) listed here; raw = divisor * count, and
+ * gat_count reports raw / divisor * multiplier. With relabeled = 1 that graph
+ * is the degree-relabelled one: vertex i is the i-th of the input's vertices
+ * ordered by degree, highest first, ties by id. Reference mode reports the
+ * reference engine's value, raw itself (divisor and multiplier 1): its
+ * restrictions are the reference's filtered ones, and with them the raw
+ * count is not the map count above (include/core/schedule.hpp:828-831). No
+ * counterpart in the reference, where Schedule::print_schedule
+ * (schedule.hpp:931-941) only prints the restrictions. */
+typedef struct gat_pattern_count_info {
+    uint64_t divisor;
+    uint64_t multiplier;
+    int32_t relabeled;
+    int32_t n_restrictions;
+    int32_t restr_first[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES];
+    int32_t restr_second[GAT_MAX_VERTEXES * GAT_MAX_VERTEXES];
+} gat_pattern_count_info;
+int gat_plan_count_info(const gat_plan *p, int32_t i, gat_pattern_count_info *out, size_t cap);
+
+/* One shard of a count: only the embeddings whose Schedule vertex 0 maps to
+ * a root in [root_begin, root_end), in vertex ids of the graph the plan runs
+ * on (for cliques in unique and labeled modes the root is the clique's
+ * highest id). raw_out[i] receives pattern i's raw sum
+ * (gat_plan_count_info); the raw sums of shards that partition [0, v_cnt)
+ * add up to what gat_count divides by the divisor. No counterpart in the
+ * reference, which counts a whole graph per call (engine.cuh:232-265): a
+ * shard is the share of one GPU when a count is split across GPUs, and
+ * bounds the work of a count on a large graph to a subset of its roots. */
+int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
+                    int32_t root_begin, int32_t root_end, uint64_t *raw_out,
+                    gat_count_stats *stats);
 void gat_plan_free(gat_plan *p);
 
 /* Text dump of a plan's compiled device program (debugging aid). Returns

@@ -39,6 +39,8 @@ def lib():
         L.orc_clique_bitmap_walg_sample.argtypes = [C.c_int32, P, P, C.c_int, C.c_int32, C.c_int32,
                                                     C.POINTER(C.c_uint64)]
         L.orc_clique_bitmap_walg_sample.restype = C.c_uint64
+        L.orc_restricted_count.argtypes = [C.c_int32, P, P, C.c_int, P, C.c_int, P, P, C.c_int32, C.c_int32]
+        L.orc_restricted_count.restype = C.c_uint64
         L.orc_relabel.argtypes = [C.c_int32, P, P, P, P]
         L.orc_relabel.restype = None
         _lib = L
@@ -94,6 +96,21 @@ def reference_count(off, adj, sched_struct):
     v = lib().orc_reference_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None,
                                   C.byref(sched_struct), C.byref(ok))
     return int(v), bool(ok.value)
+
+
+def restricted_count(off, adj, rows, restrictions, root_lo=0, root_hi=None):
+    """Injective, edge-preserving maps psi of the pattern whose vertex i is
+    adjacent to j iff bit j of rows[i], with psi(b) < psi(a) for every (a, b)
+    in restrictions, over the roots psi(0) in [root_lo, root_hi)."""
+    off, adj, n = _csr(off, adj)
+    k = len(rows)
+    r = (C.c_uint32 * max(1, k))(*rows)
+    nr = len(restrictions)
+    fa = (C.c_int32 * max(1, nr))(*[a for a, _ in restrictions])
+    sb = (C.c_int32 * max(1, nr))(*[b for _, b in restrictions])
+    hi = n if root_hi is None else root_hi
+    return int(lib().orc_restricted_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, k, r, nr,
+                                          fa, sb, int(root_lo), int(hi)))
 
 
 def relabel(off, adj):

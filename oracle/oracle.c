@@ -790,6 +790,126 @@ uint64_t orc_labeled_count(int32_t n, const int64_t *off, const int32_t *adj, co
 }
 
 /* ------------------------------------------------------------------ */
+/* Restricted maps of a Schedule-ordered pattern, by root              */
+/* ------------------------------------------------------------------ */
+/* Maps psi of the pattern (vertex i adjacent to j iff bit j of rows[i])
+ * into the graph that are injective, edge-preserving and satisfy
+ * psi(second) < psi(first) for every restriction, the direction the
+ * reference engine tests (include/engine/engine_kernel.cuh:321-334). The
+ * count does not depend on the order the vertices are placed in, so after
+ * vertex 0 (the root) the next vertex is always one with the most placed
+ * neighbours (ties: lowest index), which keeps a low-degree root from
+ * walking a hub's list twice. Level i takes the list of its placed
+ * neighbour with the fewest entries and cuts it to the open interval the
+ * restrictions between i and placed levels leave; each restriction is
+ * thereby tested once, at its later level. */
+typedef struct {
+    const int64_t *off;
+    const int32_t *adj;
+    int32_t n;
+    int k;
+    uint32_t rows[MAXN];
+    int nr;
+    int rf[MAXN * MAXN], rs[MAXN * MAXN];
+    int32_t map[MAXN];
+} rst_t;
+
+static uint64_t rst_rec(rst_t *R, int i) {
+    int j, r, anchor = -1;
+    int32_t lo = -1, hi = R->n; /* candidates w with lo < w < hi */
+    int64_t e, e0, e1;
+    uint64_t ans = 0;
+    for (j = 0; j < i; j++)
+        if ((R->rows[i] >> j) & 1u) {
+            int32_t x = R->map[j], y = anchor >= 0 ? R->map[anchor] : 0;
+            if (anchor < 0 || R->off[x + 1] - R->off[x] < R->off[y + 1] - R->off[y]) anchor = j;
+        }
+    for (r = 0; r < R->nr; r++) {
+        if (R->rs[r] == i && R->rf[r] < i && R->map[R->rf[r]] < hi) hi = R->map[R->rf[r]];
+        if (R->rf[r] == i && R->rs[r] < i && R->map[R->rs[r]] > lo) lo = R->map[R->rs[r]];
+    }
+    if (anchor >= 0) {
+        e0 = R->off[R->map[anchor]];
+        e1 = R->off[R->map[anchor] + 1];
+    } else { /* no earlier neighbour: every vertex is a candidate */
+        e0 = 0;
+        e1 = R->n;
+    }
+    for (e = e0; e < e1; e++) {
+        int32_t w = anchor >= 0 ? R->adj[e] : (int32_t)e;
+        int ok = 1;
+        if (w <= lo) continue;
+        if (w >= hi) break;
+        for (j = 0; j < i && ok; j++) {
+            if (R->map[j] == w) ok = 0;
+            else if (j != anchor && ((R->rows[i] >> j) & 1u) && !g_has_edge(R->off, R->adj, w, R->map[j])) ok = 0;
+        }
+        if (!ok) continue;
+        if (i == R->k - 1) {
+            ans++;
+        } else {
+            R->map[i] = w;
+            ans += rst_rec(R, i + 1);
+        }
+    }
+    return ans;
+}
+
+uint64_t orc_restricted_count(int32_t n, const int64_t *off, const int32_t *adj, int k, const uint32_t *rows,
+                              int nr, const int32_t *first, const int32_t *second, int32_t root_lo,
+                              int32_t root_hi) {
+    rst_t R;
+    uint64_t ans = 0;
+    int32_t v;
+    int i;
+    int order[MAXN], pos[MAXN], placed = 1, j;
+    if (k < 1 || k > MAXN || nr < 0 || nr > MAXN * MAXN) return 0;
+    R.off = off;
+    R.adj = adj;
+    R.n = n;
+    R.k = k;
+    R.nr = nr;
+    for (i = 0; i < k; i++) pos[i] = -1;
+    order[0] = 0;
+    pos[0] = 0;
+    while (placed < k) {
+        int best = -1, best_links = -1;
+        for (i = 0; i < k; i++) {
+            int links = 0;
+            if (pos[i] >= 0) continue;
+            for (j = 0; j < k; j++)
+                if (pos[j] >= 0 && (((rows[i] >> j) | (rows[j] >> i)) & 1u)) links++;
+            if (links > best_links) {
+                best = i;
+                best_links = links;
+            }
+        }
+        pos[best] = placed;
+        order[placed++] = best;
+    }
+    for (i = 0; i < k; i++) {
+        R.rows[i] = 0;
+        for (j = 0; j < k; j++)
+            if (((rows[order[i]] >> order[j]) | (rows[order[j]] >> order[i])) & 1u) R.rows[i] |= 1u << j;
+    }
+    for (i = 0; i < nr; i++) {
+        R.rf[i] = pos[first[i]];
+        R.rs[i] = pos[second[i]];
+    }
+    if (root_lo < 0) root_lo = 0;
+    if (root_hi > n) root_hi = n;
+    for (v = root_lo; v < root_hi; v++) {
+        if (k == 1) {
+            ans++;
+            continue;
+        }
+        R.map[0] = v;
+        ans += rst_rec(&R, 1);
+    }
+    return ans;
+}
+
+/* ------------------------------------------------------------------ */
 /* Reference engine semantics                                          */
 /* ------------------------------------------------------------------ */
 typedef struct {

@@ -14,6 +14,9 @@
  *                           include/implementations/array_vertex_set.cuh
  *   orc_labeled_count       labeled (injective, edge-preserving, non-induced)
  *                           embeddings by backtracking; = naive_pm's value
+ *   orc_restricted_count    the engine's raw count under a Schedule's
+ *                           restriction set, by root: restriction direction
+ *                           of include/engine/engine_kernel.cuh:321-334
  *   orc_automorphisms       schedule.hpp:173-191
  *   orc_graph_cnt           schedule.hpp:86-170
  *   orc_clique_walg         algorithmic bytes of the oriented clique walk
@@ -37,6 +40,13 @@ void orc_graph_cnt(int k, int64_t *even, int64_t *odd);
 
 uint64_t orc_naive_count(int32_t n, const int64_t *off, const int32_t *adj, const char *pattern);
 uint64_t orc_labeled_count(int32_t n, const int64_t *off, const int32_t *adj, const char *pattern);
+/* Maps psi of a k-vertex pattern in Schedule order (bit j of rows[i] = edge
+ * i-j) that are injective, edge-preserving and satisfy
+ * psi(second[r]) < psi(first[r]) for every restriction r, counted over the
+ * roots psi(0) in [root_lo, root_hi). */
+uint64_t orc_restricted_count(int32_t n, const int64_t *off, const int32_t *adj, int k, const uint32_t *rows,
+                              int nr, const int32_t *first, const int32_t *second, int32_t root_lo,
+                              int32_t root_hi);
 /* Reference engine semantics for a schedule built by orc_schedule. Returns
  * the raw long long the reference would print, as uint64 bits. *ok = 0 when
  * the schedule makes the reference read an undefined buffer. */

@@ -290,3 +290,52 @@ def test_clique_bitmap_walg_sample_partitions(orc):
         full = orc.clique_bitmap_walg(ro, ra, k)
         parts = [orc.clique_bitmap_walg_sample(ro, ra, k, 4, f) for f in range(4)]
         assert (sum(p[0] for p in parts), sum(p[1] for p in parts)) == full
+
+
+def brute_restricted(off, adj, rows, restrictions, lo, hi):
+    """Every injective map of the pattern's vertices, filtered (pure Python)."""
+    n, k = len(off) - 1, len(rows)
+    nb = [set(adj[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    total = 0
+    for psi in itertools.permutations(range(n), k):
+        if not lo <= psi[0] < hi:
+            continue
+        if any((rows[i] >> j) & 1 and psi[j] not in nb[psi[i]] for i in range(k) for j in range(i)):
+            continue
+        if all(psi[b] < psi[a] for a, b in restrictions):
+            total += 1
+    return total
+
+
+def rows_of(adj_str):
+    n = int(round(len(adj_str) ** 0.5))
+    return [sum(1 << j for j in range(n) if adj_str[i * n + j] == "1") for i in range(n)]
+
+
+def test_restricted_count_direction_by_hand(orc):
+    # K3 with psi(1) < psi(0) and psi(2) < psi(1): only psi = (2, 1, 0), rooted at 2
+    off, adj = complete(3)
+    rows, R = rows_of(TRIANGLE), [(0, 1), (1, 2)]
+    assert [orc.restricted_count(off, adj, rows, R, v, v + 1) for v in range(3)] == [0, 0, 1]
+    assert orc.restricted_count(off, adj, rows, []) == 6  # |Aut(K3)| labeled maps
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_restricted_count_against_brute_force(orc, seed):
+    rng = np.random.default_rng(seed)
+    n = 8
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.55]
+    off, adj = csr_of_edges(n, edges)
+    cases = [
+        (TRIANGLE, []), (TRIANGLE, [(0, 1), (1, 2)]), (TRIANGLE, [(0, 1), (0, 2)]),
+        (P3, [(0, 2)]),
+        ("001001110", [(0, 1)]),  # path 0-2-1: vertex 1 has no earlier neighbour
+        (K4, [(0, 1), (1, 2), (2, 3)]), (K4, [(2, 3)]),
+        (HOUSE, [(1, 2)]), (HOUSE, []),
+        (C5, [(2, 3), (0, 3), (0, 1), (0, 2), (0, 4)]), (C5, [(3, 0), (4, 1)]),
+    ]
+    for pat, R in cases:
+        rows = rows_of(pat)
+        for lo, hi in ((0, n), (0, 3), (3, 8), (2, 5), (7, 8), (4, 4)):
+            want = brute_restricted(off, adj, rows, R, lo, hi)
+            assert orc.restricted_count(off, adj, rows, R, lo, hi) == want, (pat, R, lo, hi)

@@ -1,0 +1,63 @@
+"""Shards of a count, host side (CPU only): what gat_plan_count_info reports
+and the oracle's restricted count. Over a partition of the roots, the
+oracle's restricted counts under a plan's ordered pattern and restriction
+set add up to divisor * unique count (labeled count / |Aut|), for every
+connected 3- to 5-vertex pattern in unique and labeled modes; the argument
+checks of gat_count_shard run before any device work."""
+import pytest
+
+from patterns import K4, K5, TRIANGLE, connected_patterns
+
+
+def plan_pattern(plan, i):
+    info = plan.count_info(i)
+    k = plan.schedule(i).data.v_cnt
+    return list(plan.schedule(i).data.adj_rows[:k]), info
+
+
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_shard_partition_sums_to_divisor_times_unique(gat, orc, k):
+    g = gat.Graph.rmat(6, 8, 0.57, 0.19, 0.19, 3)
+    off, adj = g.csr()
+    roff, radj = orc.relabel(off, adj)
+    n = len(off) - 1
+    cuts = [0, n // 7, n // 2, n - 3, n]
+    for p in connected_patterns(k):
+        unique = orc.unique_count(off, adj, p)
+        for mode in ("unique", "labeled"):
+            plan = gat.Plan(g, [p], mode)
+            rows, info = plan_pattern(plan, 0)
+            assert info["relabeled"]
+            assert info["multiplier"] == (1 if mode == "unique" else orc.automorphisms(p))
+            parts = [orc.restricted_count(roff, radj, rows, info["restrictions"], cuts[i], cuts[i + 1])
+                     for i in range(len(cuts) - 1)]
+            assert sum(parts) == info["divisor"] * unique, (p, mode, parts)
+
+
+def test_clique_roots_are_the_top_vertex(gat, orc):
+    """For cliques every restriction chain ends at vertex 0: psi(0) is the
+    clique's highest id, so a shard holds the cliques whose top vertex it
+    holds (what the bitmap store counts per vertex)."""
+    g = gat.Graph.rmat(6, 8, 0.57, 0.19, 0.19, 4)
+    off, adj = g.csr()
+    roff, radj = orc.relabel(off, adj)
+    n = len(off) - 1
+    for p, kk in ((TRIANGLE, 3), (K4, 4), (K5, 5)):
+        plan = gat.Plan(g, [p], "unique")
+        rows, info = plan_pattern(plan, 0)
+        assert info["divisor"] == 1
+        for v in range(0, n, 5):
+            want = orc.clique_bitmap_walg_sample(roff, radj, kk, n, v)[0]
+            assert orc.restricted_count(roff, radj, rows, info["restrictions"], v, v + 1) == want
+
+
+def test_shard_argument_checks(gat):
+    from graphautotune_amd._lib import GAT_EINVAL, GatError
+    g = gat.Graph.erdos_renyi(40, 0.2, 1)
+    plan = gat.Plan(g, [K4], "unique")
+    for lo, hi in ((-1, 3), (5, 4), (0, 41)):
+        with pytest.raises(GatError) as e:
+            plan.count_shard(lo, hi)
+        assert e.value.code == GAT_EINVAL
+    with pytest.raises(GatError):
+        plan.count_info(1)
