@@ -98,6 +98,63 @@ def cpu_baseline(g, total_walg, budget_s):
     }
 
 
+IEP_PATTERNS = {"house": "0100110110010110110010100", "c5": "0100110100010100010110010"}
+
+
+def make_graph(gat, name):
+    """rmatS: R-MAT scale S, edge factor 16, (.57,.19,.19,.05), seed 20;
+    clN: Chung-Lu, N vertices, mean degree 24, exponent 2.1, degree cap
+    50,000, seed 21."""
+    if name.startswith("rmat"):
+        return gat.Graph.rmat(int(name[4:]), 16, 0.57, 0.19, 0.19, 20), \
+            "R-MAT scale %s, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 20" % name[4:]
+    if name.startswith("cl"):
+        return gat.Graph.chung_lu(int(name[2:]), 24.0, 2.1, 50000, 21), \
+            "Chung-Lu, %s vertices, mean degree 24, exponent 2.1, degree cap 50,000, seed 21" % name[2:]
+    raise SystemExit("unknown graph %r (rmatS or clN)" % name)
+
+
+def iep_line(gat, args):
+    """Auxiliary line: one 5-vertex IEP pattern on the array store (default
+    config, hub bitmaps on), unique mode. Roofline of the longer of the main
+    (LDS) and overflow (HBM) launches, as for the headline."""
+    g, desc = make_graph(gat, args.graph)
+    E = g.undirected_edges()
+    plan = gat.Plan(g, [IEP_PATTERNS[args.iep]], "unique")
+    cfg = json.loads(args.config) if args.config else {}
+    for _ in range(args.warmup):
+        plan.count(cfg)
+    _, acc = plan.count(cfg, account=True)
+    counts, secs, (mains, seconds), walls, st = timed(plan, cfg, args.steps)
+    t = statistics.median(secs)
+    launches = [("gatk::engine_kernel<false>", statistics.median(mains), acc["alg_bytes_main"]),
+                ("gatk::engine_kernel<true>", statistics.median(seconds), acc["alg_bytes_second"])]
+    name, ms, walg = max(launches, key=lambda x: x[1])
+    achieved = walg / (ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_iep.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f).get("%s-%s" % (args.graph, args.iep), {})
+        if tr.get("config") == cfg:
+            traffic = tr.get("bytes_per_launch", {}).get(name)
+    print(json.dumps({
+        "metric": "exact-count throughput (edges/s), 5-vertex IEP pattern, array store",
+        "value": E / t, "unit": "edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sum(walls) / len(walls) * 1e3, "higher_is_better": True, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "%s; exact unique %s count" % (desc, args.iep), "vertices": g.v_cnt(),
+                   "undirected_edges": E, "pattern": IEP_PATTERNS[args.iep], "kernel_config": cfg},
+        "counts": counts, "device_ms": t * 1e3, "overflow_tasks": st["overflow_tasks"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "hbm_achieved": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+                     "alg_bytes_per_launch": walg, "kernel": name, "kernel_ms": ms,
+                     "launches_ms": {k: v for k, v, _ in launches},
+                     "step_alg_bytes": acc["alg_bytes"], "step_achieved": acc["alg_bytes"] / t / 1e9},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +169,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB (3 timed launches) and exit; "
                          "calibrates the HBM byte counters")
+    ap.add_argument("--iep", choices=sorted(IEP_PATTERNS), default=None,
+                    help="auxiliary line: one 5-vertex IEP pattern on the array store instead of K4 + K5")
+    ap.add_argument("--graph", default="rmat14", help="with --iep: rmatS or clN (see make_graph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -125,6 +185,9 @@ def main():
         gbps = C.c_double()
         check(lib().gat_copy_probe(args.copy_probe << 20, 3, C.byref(gbps)))
         print(json.dumps({"copy_probe_mib": args.copy_probe, "launches": 4, "gbps": gbps.value}), flush=True)
+        return
+    if args.iep:
+        iep_line(gat, args)
         return
     try:
         import torch
