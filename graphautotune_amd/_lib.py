@@ -78,12 +78,12 @@ class KernelConfig(C.Structure):
         ("max_regs", C.c_int32),
         ("vertex_store", C.c_int32),
         ("hub_degree", C.c_int32),
-        ("reserved", C.c_int32),
+        ("split_us", C.c_int32),
     ]
 
     KNOBS = ("block_waves", "waves_per_cu", "queue_chunk", "slot_capacity", "leaf_mode",
              "intersect_strategy", "gallop_ratio", "search_linear_threshold", "max_regs", "vertex_store",
-             "hub_degree")
+             "hub_degree", "split_us")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.KNOBS}
@@ -109,7 +109,8 @@ class CountStats(C.Structure):
         ("grid_blocks", C.c_int32),
         ("block_threads", C.c_int32),
         ("lds_bytes", C.c_int32),
-        ("reserved", C.c_int32),
+        ("split_tasks", C.c_int32),
+        ("split_overflow", C.c_int32),
     ]
 
 

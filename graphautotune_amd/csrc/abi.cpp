@@ -288,6 +288,7 @@ int gat_default_config(gat_kernel_config *out) {
     out->search_linear_threshold = 0;
     out->max_regs = 0;
     out->hub_degree = 256;
+    out->split_us = 2000;
     return GAT_OK;
 }
 

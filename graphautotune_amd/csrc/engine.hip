@@ -34,6 +34,16 @@ namespace gatk {
 
 constexpr int W = 64;
 
+// A split subtask: candidates [lo, hi) of the loop of node `node` (at depth
+// `depth` >= 2) below root edge `edge` (directed edge index: v0 = its row,
+// v1 = its entry), with the candidates of depths 2 .. depth-1 taken from
+// loop positions pos[2 ..]. lo >= hi marks an empty or cancelled entry.
+struct SplitTask {
+    long long edge;
+    int node, depth, lo, hi;
+    int pos[GATP_MAX_DEPTH];
+};
+
 struct KArgs {
     const long long *off;
     const int *adj;
@@ -63,7 +73,21 @@ struct KArgs {
     const int *hub_map;
     long long hub_words;  // words per row: (n + 31) / 32
     int n_hubs;
-    int pad_;
+    // Splitting: a task that has run longer than split_ticks of the wall
+    // clock while the queue it came from is drained hands the rest of its
+    // shallowest unfinished loop at depth >= 2 to subs[out_base ..] (count in
+    // *out_n, at most out_cap used). The main launch makes level-1 subtasks;
+    // the level-1 subtask launch (sub_mode 1) runs them and may split once
+    // more; the level-2 launch (sub_mode 2) runs those without splitting.
+    // Subtasks that outgrow LDS are copied to the third list
+    // (subs[sovf_base ..], count *sovf_n), which also takes the splits of the
+    // HBM launch; a last subtask launch with HBM buffers (sub_mode 3) runs it.
+    int sub_mode;
+    SplitTask *subs;
+    unsigned long long *in_n, *out_n, *sovf_n;
+    long long in_base, out_base, sovf_base;
+    long long in_cap, out_cap, sovf_cap;
+    long long split_ticks;  // <= 0: no splitting
 };
 
 struct WaveState {
@@ -79,8 +103,15 @@ struct WaveState {
     int leafcnt[GATP_MAX_LEAF_OPS][W];
     unsigned long long bytes;   // committed W_alg of this wave
     unsigned long long ebytes;  // W_alg of the work not yet committed
+    long long t0;               // wall clock at the start of the task
+    long long rb;               // first edge of the task's root v0
+    long long sub_edge;         // root edge of the running subtask, or -1
+    long long don_base[8];      // subtask blocks donated since the last commit
+    int don_cnt[8];
+    int n_don;
+    int base_depth;             // frames exist from this depth on (2, or a subtask's depth)
+    int path_pos[GATP_MAX_DEPTH + 1];  // a subtask's positions at depths 2 .. base_depth-1
     int abort;
-    int pad;
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -293,7 +324,7 @@ __device__ __forceinline__ bool in_lds(const Ctx &C, int slot) {
 
 // Evaluate one op for the wave-uniform candidate c at depth d.
 template <bool GS>
-__device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool dup) {
+__device__ __forceinline__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool dup) {
     const KArgs &a = C.a;
     WaveState *S = C.S;
     const int t = uni(group_bound(S, o.groups, o.ngroups, d, c));
@@ -393,6 +424,96 @@ __device__ void exec_op_uniform(const Ctx &C, const GatOp &o, int d, int c, bool
     }
 }
 
+// Subtask blocks handed on since the last commit are emptied when the work
+// they came from is abandoned: the HBM launch re-runs that work whole.
+__device__ __noinline__ void cancel_donations(const KArgs &a, WaveState *S, int ln) {
+    const int nd = S->n_don;
+    for (int i = 0; i < nd; i++) {
+        const long long b = S->don_base[i];
+        const int cnt = S->don_cnt[i];
+        for (int k = ln; k < cnt; k += W) {
+            a.subs[b + k].lo = 0;
+            a.subs[b + k].hi = 0;
+        }
+    }
+    wsync();
+    if (ln == 0) S->n_don = 0;
+    wsync();
+}
+
+// Split only in the tail: the task has outrun its budget and the queue it
+// came from has nothing left for idle waves.
+template <bool GS>
+__device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
+    if (a.split_ticks <= 0 || a.sub_mode >= 2) return false;
+    if (uni64(wall_clock64()) - S->t0 <= a.split_ticks) return false;
+    if (GS) {  // the HBM launch: its overflow list is drained
+        const unsigned long long made = __atomic_load_n(a.ovf_n, __ATOMIC_RELAXED);
+        const unsigned long long total = made < (unsigned long long)a.ovf_cap ? made : (unsigned long long)a.ovf_cap;
+        return __atomic_load_n(&a.queue[1], __ATOMIC_RELAXED) >= total;
+    }
+    if (a.sub_mode == 0)
+        return (long long)__atomic_load_n(&a.queue[0], __ATOMIC_RELAXED) >= a.e_hi - a.e_lo;
+    const unsigned long long made = __atomic_load_n(a.in_n, __ATOMIC_RELAXED);
+    const unsigned long long total = made < (unsigned long long)a.in_cap ? made : (unsigned long long)a.in_cap;
+    return __atomic_load_n(&a.queue[2], __ATOMIC_RELAXED) >= total;
+}
+
+// Hands candidates [lo, hi) of the depth-d frame's node to the next split
+// launch, in at most 64 chunks of at least min_chunk. False, with nothing
+// handed on, when the wave's record of donations or the list is full
+// (reserved entries past the end of the list are left empty).
+__device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int hi, int min_chunk) {
+    const KArgs &a = C.a;
+    WaveState *S = C.S;
+    if (S->n_don >= 8) return false;
+    const long long edge = S->sub_edge >= 0 ? S->sub_edge : S->rb + (S->f_cur[1] - 1);
+    const int r = hi - lo;
+    const int chunk = max(min_chunk, (r + W - 1) / W);
+    const int parts = (r + chunk - 1) / chunk;
+    unsigned long long b = 0;
+    if (C.ln == 0) b = atomicAdd(a.out_n, (unsigned long long)parts);
+    const long long base = uni64((long long)__shfl(b, 0));
+    const long long room = a.out_cap - base;
+    const int fit = room <= 0 ? 0 : (room < parts ? (int)room : parts);
+    for (int k = C.ln; k < fit; k += W) {
+        SplitTask *t = a.subs + a.out_base + base + k;
+        t->edge = edge;
+        t->node = node;
+        t->depth = d;
+        t->lo = fit == parts ? lo + k * chunk : 0;
+        t->hi = fit == parts ? min(hi, lo + (k + 1) * chunk) : 0;
+        for (int j = 2; j < d; j++) t->pos[j] = j < S->base_depth ? S->path_pos[j] : S->f_cur[j] - 1;
+    }
+    if (fit < parts) return false;
+    if (C.ln == 0) {
+        S->don_base[S->n_don] = a.out_base + base;
+        S->don_cnt[S->n_don] = parts;
+        S->n_don++;
+    }
+    wsync();
+    return true;
+}
+
+// Donates the rest of the shallowest frame (depth >= base_depth, up to d)
+// that has two or more candidates left; true if it did. The frame is closed
+// by lowering its end to its cursor: f_cur[k] - 1 stays the position of the
+// candidate being walked at depth k, which later donations from deeper
+// frames record as their path.
+template <bool GS>
+__device__ __noinline__ bool split_shallowest(const Ctx &C, int d) {
+    WaveState *S = C.S;
+    for (int k = uni(S->base_depth); k <= d; k++) {
+        const int cur = uni(S->f_cur[k]), fh = uni(S->f_hi[k]);
+        if (fh - cur < 2) continue;
+        if (!donate(C, k, uni(S->f_node[k]), cur, fh, C.P.nodes[uni(S->f_node[k])].leaf ? W : 1)) return false;
+        if (C.ln == 0) S->f_hi[k] = cur;
+        wsync();
+        return true;
+    }
+    return false;
+}
+
 // Value of one IEP factor. Per-lane candidate c sits at depth d; leaf
 // counters are read for this lane.
 template <bool GS>
@@ -435,8 +556,10 @@ __device__ __forceinline__ unsigned long long term_value(const Ctx &C, const Gat
     return local;
 }
 
+// replay: rebuild the node's sets for a split subtask without adding its
+// terms again (the donor counted them).
 template <bool GS>
-__device__ bool visit(const Ctx &C, const GatNode &N, int d, int c) {
+__device__ __forceinline__ bool visit(const Ctx &C, const GatNode &N, int d, int c, bool replay = false) {
     WaveState *S = C.S;
     if (C.ln == 0) S->val[d] = c;
     wsync();
@@ -446,7 +569,7 @@ __device__ bool visit(const Ctx &C, const GatNode &N, int d, int c) {
         wsync();
         if (S->abort) return false;
     }
-    for (int k = 0; k < N.term_count; k++) {
+    for (int k = 0; k < N.term_count && !replay; k++) {
         const GatTerm &T = C.P.terms[N.term_begin + k];
         const unsigned long long v = term_value<GS>(C, T, d, c);
         if (C.ln == 0) S->task_acc[T.pattern] += v;
@@ -456,8 +579,11 @@ __device__ bool visit(const Ctx &C, const GatNode &N, int d, int c) {
 }
 
 // All candidates of a leaf frame, flattened across lanes.
+// With can_split, a wave that is due to split (want_split) hands the
+// batches after the current one to the next split launch.
 template <bool GS>
-__device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
+__device__ __forceinline__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi, bool can_split = false,
+                                     int node = -1) {
     const KArgs &a = C.a;
     WaveState *S = C.S;
     const int ln = C.ln;
@@ -568,6 +694,7 @@ __device__ void leaf(const Ctx &C, const GatNode &N, int d, int cur, int hi) {
             if (ln == 0) S->task_acc[Tm.pattern] += v;
         }
         wsync();
+        if (can_split && hi - (cb + W) >= 2 * W && want_split<GS>(a, S) && donate(C, d, node, cb + W, hi, W)) break;
     }
     if (a.account) {
         bytes = wave_sum64(bytes);
@@ -604,9 +731,11 @@ __device__ __forceinline__ void commit_pending(WaveState *S, int np, int ln) {
     if (ln == 0) {
         S->bytes += S->ebytes;
         S->ebytes = 0;
+        S->n_don = 0;  // what this work handed on is final too
     }
     wsync();
 }
+
 __device__ __forceinline__ void drop_pending(WaveState *S, int np, int ln) {
     for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
     if (ln == 0) {
@@ -622,7 +751,7 @@ __device__ __forceinline__ void drop_pending(WaveState *S, int np, int ln) {
 // launch and the walk goes on with the next edge; otherwise the whole task
 // is abandoned and re-run there.
 template <bool GS>
-__device__ void run_task(const Ctx &C, long long e0, long long e1) {
+__device__ __forceinline__ void run_task(const Ctx &C, long long e0, long long e1) {
     const KArgs &a = C.a;
     const GatPlanDev &P = C.P;
     WaveState *S = C.S;
@@ -639,7 +768,12 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
         const int v0 = uni(lo);
         const long long rb = a.off[v0], re = a.off[v0 + 1];
         const long long se = e1 < re ? e1 : re;
-        if (C.ln == 0) S->val[0] = v0;
+        if (C.ln == 0) {
+            S->val[0] = v0;
+            S->rb = rb;
+            S->sub_edge = -1;
+            S->base_depth = 2;
+        }
         wsync();
         for (int k = 0; k < root.op_count; k++) exec_op_uniform<GS>(C, P.ops[root.op_begin + k], 0, v0, false);
         wsync();
@@ -650,11 +784,14 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
             const int node = uni(S->f_node[d]);
             const GatNode &N = P.nodes[node];
             bool exhausted = false;
+            // loops at depth >= 2 may be handed to a split launch
+            const bool can_split = d >= 2;
             if (N.leaf) {
-                leaf<GS>(C, N, d, uni(S->f_cur[d]), uni(S->f_hi[d]));
+                leaf<GS>(C, N, d, uni(S->f_cur[d]), uni(S->f_hi[d]), can_split, node);
                 exhausted = true;
             } else {
                 const int cur = uni(S->f_cur[d]), fh = uni(S->f_hi[d]);
+                if (can_split && cur < fh && want_split<GS>(a, S) && split_shallowest<GS>(C, d)) continue;
                 if (cur < fh) {
                     const int c = uni(with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[cur]; }));
                     if (C.ln == 0) S->f_cur[d] = cur + 1;
@@ -668,6 +805,7 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
                             const long long edge = rb + (uni(S->f_cur[1]) - 1);
                             if ((long long)idx < a.ovf_cap) a.ovf[idx] = edge; else atomicOr(a.err, 2u);
                         }
+                        cancel_donations(a, S, C.ln);
                         drop_pending(S, P.n_patterns, C.ln);
                         d = 1;
                         continue;
@@ -692,6 +830,84 @@ __device__ void run_task(const Ctx &C, long long e0, long long e1) {
     if (edge_grain) commit_pending(S, P.n_patterns, C.ln);
 }
 
+// Split subtask *g (in subs[]): the sets of the root and of the nodes on its
+// path are rebuilt (neither counted nor accounted again), then the walk
+// covers candidates [lo, hi) of its node and their subtrees only. A set that
+// outgrows its LDS buffer leaves S->abort set (the HBM launch re-runs it).
+template <bool GS>
+__device__ __forceinline__ void run_sub(const Ctx &C, const SplitTask *g) {
+    const KArgs &a = C.a;
+    const GatPlanDev &P = C.P;
+    WaveState *S = C.S;
+    const long long e = uni64(g->edge);
+    const int sub_node = uni(g->node), D = uni(g->depth), sub_lo = uni(g->lo), sub_hi = uni(g->hi);
+    int lo = 0, hi = a.n;  // off[lo] <= e < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int v0 = uni(lo);
+    const long long rb = a.off[v0];
+    const unsigned long long eb = S->ebytes;
+    if (C.ln == 0) {
+        S->val[0] = v0;
+        S->rb = rb;
+        S->sub_edge = e;
+        S->base_depth = D;
+    }
+    for (int j = 2 + C.ln; j < D; j += W) S->path_pos[j] = g->pos[j];
+    wsync();
+    const GatNode &root = P.nodes[0];
+    for (int k = 0; k < root.op_count; k++) exec_op_uniform<GS>(C, P.ops[root.op_begin + k], 0, v0, false);
+    wsync();
+    bool ok = true;
+    for (int j = 1; j < D && ok; j++) {
+        int nj = sub_node;  // the ancestor of the subtask's node at depth j
+        for (int t = D; t > j; t--) nj = P.nodes[nj].parent;
+        const GatNode &Nj = P.nodes[nj];
+        const int pos = j == 1 ? (int)(e - rb) : uni(S->path_pos[j]);
+        const int vj = uni(with_slot<GS>(C, Nj.loop_slot, [&](auto p) { return p[pos]; }));
+        ok = visit<GS>(C, Nj, j, vj, true);
+    }
+    if (C.ln == 0) S->ebytes = eb;
+    wsync();
+    if (!ok) return;
+    const bool can_split = !GS && a.sub_mode == 1;
+    push_frame<GS>(C, D, sub_node, sub_lo, sub_hi);
+    int d = D;
+    while (d >= D) {
+        const int node = uni(S->f_node[d]);
+        const GatNode &N = P.nodes[node];
+        bool exhausted = false;
+        if (N.leaf) {
+            leaf<GS>(C, N, d, uni(S->f_cur[d]), uni(S->f_hi[d]), can_split, node);
+            exhausted = true;
+        } else {
+            const int cur = uni(S->f_cur[d]), fh = uni(S->f_hi[d]);
+            if (can_split && cur < fh && want_split<GS>(a, S) && split_shallowest<GS>(C, d)) continue;
+            if (cur < fh) {
+                const int c = uni(with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[cur]; }));
+                if (C.ln == 0) S->f_cur[d] = cur + 1;
+                wsync();
+                if (N.excl && equals_any(S, N.excl, d, c)) continue;
+                if (!visit<GS>(C, N, d, c)) return;
+                if (N.child_count > 0) {
+                    d++;
+                    push_frame<GS>(C, d, N.child_begin, 0, INT_MAX);
+                }
+                continue;
+            }
+            exhausted = true;
+        }
+        if (exhausted) {
+            if (d > D && N.next_sibling >= 0)
+                push_frame<GS>(C, d, N.next_sibling, 0, INT_MAX);
+            else
+                d--;
+        }
+    }
+}
+
 template <bool GS>
 __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -713,34 +929,111 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
     }
     wsync();
     for (;;) {
-        long long e0, e1;
+        long long e0;  // the task: directed edges [e0, e0 + chunk)
         if (GS) {
             unsigned long long r = 0;
             if (ln == 0) r = atomicAdd(&a.queue[1], 1ull);
             r = (unsigned long long)uni64((long long)__shfl(r, 0));
             const unsigned long long total = *a.ovf_n;
             if (r >= total || (long long)r >= a.ovf_cap) break;
-            e0 = a.ovf[r];
+            e0 = uni64(a.ovf[r]);
         } else {
             unsigned long long r = 0;
             if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
             e0 = a.e_lo + uni64((long long)__shfl(r, 0));
             if (e0 >= a.e_hi) break;
         }
-        e1 = e0 + a.chunk < a.e_hi ? e0 + a.chunk : a.e_hi;
         for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
-        if (ln == 0) S->abort = 0;
+        if (ln == 0) {
+            S->abort = 0;
+            S->n_don = 0;
+            S->t0 = wall_clock64();
+        }
         wsync();
-        run_task<GS>(C, e0, e1);
+        run_task<GS>(C, e0, e0 + a.chunk < a.e_hi ? e0 + a.chunk : a.e_hi);
         wsync();
         if (S->abort) {
             if (GS) {
                 if (ln == 0) atomicOr(a.err, 1u);  // cannot happen: HBM buffers hold max degree
-            } else if (ln == 0) {
-                const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
-                if ((long long)idx < a.ovf_cap) a.ovf[idx] = e0; else atomicOr(a.err, 2u);
+            } else {
+                cancel_donations(a, S, ln);
+                if (ln == 0) {
+                    const unsigned long long idx = atomicAdd(a.ovf_n, 1ull);
+                    if ((long long)idx < a.ovf_cap) a.ovf[idx] = e0; else atomicOr(a.err, 2u);
+                }
             }
             if (ln == 0) S->ebytes = 0;  // the HBM launch does this task again
+        } else {
+            for (int i = ln; i < np; i += W) S->wave_acc[i] += S->task_acc[i];
+            if (ln == 0) {
+                S->bytes += S->ebytes;
+                S->ebytes = 0;
+            }
+        }
+        wsync();
+    }
+    for (int i = ln; i < np; i += W)
+        if (S->wave_acc[i]) atomicAdd(&a.counts[i], S->wave_acc[i]);
+    if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
+}
+
+// Subtask launches: list sub_mode (1, 2: LDS buffers; 3: HBM buffers). A
+// subtask that outgrows LDS is copied to the third list for the last launch.
+template <bool GS>
+__global__ void __launch_bounds__(256) engine_sub_kernel(KArgs a, int wave_bytes) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int wid = threadIdx.x / W, ln = threadIdx.x % W;
+    const int wpb = blockDim.x / W;
+    WaveState *S = reinterpret_cast<WaveState *>(smem + (size_t)wid * wave_bytes);
+    const GatPlanDev &P = *a.plan;
+    int *store;
+    if (GS)
+        store = a.gstore + (long long)(blockIdx.x * wpb + wid) * P.n_store * (long long)a.cap;
+    else
+        store = reinterpret_cast<int *>(smem + (size_t)wid * wave_bytes + sizeof(WaveState));
+    Ctx C{a, P, S, store, ln};
+    const int np = P.n_patterns;
+    for (int i = ln; i < np; i += W) S->wave_acc[i] = 0;
+    if (ln == 0) {
+        S->bytes = 0;
+        S->ebytes = 0;
+    }
+    wsync();
+    for (;;) {
+        unsigned long long r = 0;
+        if (ln == 0) r = atomicAdd(&a.queue[1 + a.sub_mode], 1ull);
+        r = (unsigned long long)uni64((long long)__shfl(r, 0));
+        const unsigned long long made = *a.in_n;
+        const unsigned long long total = made < (unsigned long long)a.in_cap ? made : (unsigned long long)a.in_cap;
+        if (r >= total) break;
+        const SplitTask *g = a.subs + a.in_base + (long long)r;
+        for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
+        if (ln == 0) {
+            S->abort = 0;
+            S->n_don = 0;
+            S->t0 = wall_clock64();
+        }
+        wsync();
+        if (uni(g->lo) < uni(g->hi)) run_sub<GS>(C, g);
+        wsync();
+        if (S->abort) {
+            if (GS) {
+                if (ln == 0) atomicOr(a.err, 1u);  // cannot happen: HBM buffers hold max degree
+            } else {
+                cancel_donations(a, S, ln);
+                unsigned long long idx = 0;
+                if (ln == 0) idx = atomicAdd(a.sovf_n, 1ull);
+                idx = (unsigned long long)uni64((long long)__shfl(idx, 0));
+                if ((long long)idx < a.sovf_cap) {
+                    SplitTask *t = a.subs + a.sovf_base + idx;
+                    for (int k = ln; k < (int)(sizeof(SplitTask) / sizeof(int)); k += W)
+                        reinterpret_cast<int *>(t)[k] = reinterpret_cast<const int *>(g)[k];
+                } else if (ln == 0) {
+                    atomicOr(a.err, 2u);
+                }
+                if (ln == 0) atomicAdd(&a.ovf_n[2], 1ull);  // statistics
+            }
+            if (ln == 0) S->ebytes = 0;
         } else {
             for (int i = ln; i < np; i += W) S->wave_acc[i] += S->task_acc[i];
             if (ln == 0) {
@@ -1183,6 +1476,7 @@ struct DeviceInfo {
     int cus = 0;
     size_t lds_per_block = 0;
     size_t lds_per_cu = 0;
+    long long wall_khz = 0;  // wall_clock64() ticks per millisecond
     char arch[64] = {};
 };
 
@@ -1206,6 +1500,9 @@ static int device_info(DeviceInfo *out) {
         info.cus = p.multiProcessorCount;
         info.lds_per_block = p.sharedMemPerBlock;
         info.lds_per_cu = p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : p.sharedMemPerBlock;
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0) khz = 100000;
+        info.wall_khz = khz;
         std::snprintf(info.arch, sizeof(info.arch), "%s", p.gcnArchName);
         if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
             set_error("libgat is built for gfx950; device is %s", p.gcnArchName);
@@ -1255,6 +1552,7 @@ struct PlanDevice {
     unsigned long long *counts = nullptr, *queue = nullptr, *ovf_n = nullptr, *bytes = nullptr;
     unsigned int *err = nullptr;
     long long *ovf = nullptr;
+    gatk::SplitTask *subs = nullptr;  // split subtasks (array store)
     long long *tier[3] = {};  // clique vertices of the workgroup tiers
     long long ovf_cap = 0;
     int *gstore = nullptr;
@@ -1278,7 +1576,7 @@ void destroy_plan_device(PlanDevice *d) {
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
-                    (void *)d->err, (void *)d->ovf, (void *)d->tier[0], (void *)d->tier[1], (void *)d->tier[2],
+                    (void *)d->err, (void *)d->ovf, (void *)d->subs, (void *)d->tier[0], (void *)d->tier[1], (void *)d->tier[2],
                     (void *)d->gstore, (void *)d->grows,
                     (void *)d->goff})
         if (p) (void)hipFree(p);
@@ -1354,6 +1652,11 @@ static int ensure_hubs(const Graph &g, DeviceGraph *dg, int thr) {
     return GAT_OK;
 }
 
+// Split subtasks: capacity of each level's list, and the default time a
+// task runs in the tail of a launch before it hands on its unfinished loops.
+constexpr long long kSubCap = 1ll << 20;
+constexpr int kSplitUs = 2000;
+
 struct Launch {
     gatk::KArgs a;
     dim3 grid, block;
@@ -1395,6 +1698,7 @@ static void fill_defaults(gat_kernel_config *c) {
     if (c->gallop_ratio <= 0) c->gallop_ratio = 32;
     if (c->search_linear_threshold < 0) c->search_linear_threshold = 0;
     if (c->hub_degree == 0) c->hub_degree = 256;
+    if (c->split_us == 0) c->split_us = kSplitUs;
 }
 
 static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
@@ -1416,8 +1720,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         HIPCHECK(hipMalloc(&d->plan, sizeof(GatPlanDev)));
         HIPCHECK(hipMemcpy(d->plan, &pd, sizeof(GatPlanDev), hipMemcpyHostToDevice));
         HIPCHECK(hipMalloc(&d->counts, sizeof(unsigned long long) * GATP_MAX_TERMS));
-        HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 4));
-        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 4));
+        HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 8));
+        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 8));
         HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 4));
         HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
         for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
@@ -1440,7 +1744,12 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         GAT_FAIL(GAT_EINFEASIBLE, "bitmap vertex store needs a plan of cliques K3..K6 in unique or labeled mode");
     // single-child roots hand over single edges (run_task), others whole tasks
     const bool edge_grain = pd.nodes[0].child_count == 1;
+    // overflow entries: edges or tasks, and split subtasks
     const long long ovf_cap = clique ? 1 : (edge_grain ? g.m + 1 : (g.m + chunk - 1) / chunk + 1);
+    const bool split = !clique && cfg.split_us > 0;
+    // subtask lists: level 1 and level 2 (kSubCap each), then the third list
+    // (LDS overflow of levels 1-2 plus the HBM launch's splits, 3 kSubCap)
+    if (split && !d->subs) HIPCHECK(hipMalloc(&d->subs, sizeof(gatk::SplitTask) * 5 * kSubCap));
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
         for (auto &t : d->tier)
@@ -1480,6 +1789,16 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     a.gallop_ratio = cfg.gallop_ratio;
     a.strategy = cfg.intersect_strategy;
     a.account = 0;
+    a.sub_mode = 0;
+    a.subs = d->subs;
+    a.in_n = nullptr;
+    a.out_n = d->ovf_n + 1;  // the main launch splits into level 1
+    a.in_base = a.out_base = 0;
+    a.in_cap = a.out_cap = kSubCap;
+    a.sovf_n = d->ovf_n + 4;
+    a.sovf_base = 2 * kSubCap;
+    a.sovf_cap = 3 * kSubCap;
+    a.split_ticks = split ? (long long)cfg.split_us * di.wall_khz / 1000 : 0;
     if (!clique && cfg.hub_degree > 0) {
         rc = ensure_hubs(g, d->dg.get(), cfg.hub_degree);
         if (rc) return rc;
@@ -1570,8 +1889,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
 static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     hipStream_t s = d->stream;
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->counts, (long long)GATP_MAX_TERMS, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 4ll, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 4ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 8ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 8ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->bytes, 4ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
     gatk::KArgs a = L.a, ao = L.ao;
@@ -1601,7 +1920,34 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
         hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1>), L.big_grid,
                            dim3(64 * gatk::CBLOCKW1), 0, s, c);
     } else {
+        if (a.split_ticks > 0) {  // split subtasks, in LDS like the main launch
+            gatk::KArgs s1 = a, s2 = a;
+            s1.sub_mode = 1;  // level 1: may split once more, into level 2
+            s1.in_n = d->ovf_n + 1;
+            s1.in_base = 0;
+            s1.out_n = d->ovf_n + 3;
+            s1.out_base = kSubCap;
+            s1.bytes = s2.bytes = d->bytes + 1;
+            s2.sub_mode = 2;  // level 2: no further splitting
+            s2.in_n = d->ovf_n + 3;
+            s2.in_base = kSubCap;
+            s2.out_n = nullptr;
+            hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, s1, L.wave_bytes);
+            hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, s2, L.wave_bytes);
+        }
+        ao.out_n = d->ovf_n + 4;  // the HBM launch splits into the third list
+        ao.out_base = 2 * kSubCap;
+        ao.out_cap = 3 * kSubCap;
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+        if (ao.split_ticks > 0) {
+            gatk::KArgs s3 = ao;
+            s3.sub_mode = 3;  // HBM buffers, no further splitting
+            s3.in_n = d->ovf_n + 4;
+            s3.in_base = 2 * kSubCap;
+            s3.in_cap = 3 * kSubCap;
+            s3.out_n = nullptr;
+            hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, s3, L.ovf_wave_bytes);
+        }
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[2], s));
@@ -1615,7 +1961,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
                   bool raw_sums = false) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf[4] = {0, 0, 0, 0}, bytes[3] = {0, 0, 0};
+    unsigned long long raw[GATP_MAX_TERMS], ovf[8] = {}, bytes[3] = {0, 0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
@@ -1644,6 +1990,12 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->second_kernel_ms = ms_second;
         }
         st->overflow_tasks = L.clique ? ovf[3] : ovf[0];
+        st->split_tasks = L.clique ? 0
+                                   : (int32_t)(std::min<unsigned long long>(ovf[1], (unsigned long long)kSubCap) +
+                                               std::min<unsigned long long>(ovf[3], (unsigned long long)kSubCap) +
+                                               std::min<unsigned long long>(ovf[4], (unsigned long long)(3 * kSubCap)) -
+                                               ovf[2]);
+        st->split_overflow = L.clique ? 0 : (int32_t)ovf[2];
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
         st->alg_bytes_second = bytes[1];

@@ -39,7 +39,7 @@ static bool set_knob(gat_kernel_config *c, const std::string &k, int v) {
                  {"search_linear_threshold", &c->search_linear_threshold},
                  {"max_regs", &c->max_regs},
                  {"vertex_store", &c->vertex_store},
-                 {"hub_degree", &c->hub_degree}};
+                 {"hub_degree", &c->hub_degree}, {"split_us", &c->split_us}};
     for (auto &kn : knobs)
         if (k == kn.name) {
             *kn.field = v;

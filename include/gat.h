@@ -45,7 +45,7 @@ extern "C" {
 #define GAT_MAX_PREFIXS 16
 #define GAT_MAX_IEP_GROUPS 64 /* schedule.hpp:962 has 50 */
 
-/* Counting semantics (see DESIGN.md "Count semantics").
+/* Counting semantics.
  *   GAT_MODE_UNIQUE    : number of distinct non-induced subgraphs of the
  *                        data graph isomorphic to the pattern.
  *   GAT_MODE_REFERENCE : the value the reference engine computes for the
@@ -132,27 +132,33 @@ typedef struct gat_kernel_config {
                                  get a dense neighbourhood bitmap in HBM that
                                  replaces binary search in their lists
                                  (0 = 256, negative = none)                   */
-    int32_t reserved;
+    int32_t split_us;         /* array store: a task still running after this
+                                 many microseconds hands the rest of its
+                                 depth-2 loop to a split launch in up to 64
+                                 parts (0 = 2000, negative = never); counts
+                                 and W_alg do not change                      */
 } gat_kernel_config;
 
 /* Per-call statistics filled by gat_count (any pointer may be NULL). */
 typedef struct gat_count_stats {
     double seconds;           /* device time of the count (hipEvents)        */
     double main_kernel_ms;    /* duration of the main engine launch          */
-    double second_kernel_ms;  /* duration of the second launch: the HBM
-                                 overflow launch (array store) or workgroup
-                                 tier 0 (bitmap store)                        */
+    double second_kernel_ms;  /* array store: the split and HBM overflow
+                                 launches after the main one; bitmap store:
+                                 workgroup tier 0                             */
     uint64_t overflow_tasks;  /* array store: tasks re-run with HBM-resident
                                  prefix sets; bitmap store: vertices passed to
                                  the workgroup tiers                          */
     uint64_t alg_bytes;       /* W_alg of the whole count (GAT_FLAG_ACCOUNT)  */
     uint64_t alg_bytes_main;  /* W_alg of the main launch alone              */
-    uint64_t alg_bytes_second;/* W_alg of the second launch alone            */
+    uint64_t alg_bytes_second;/* W_alg of the launches second_kernel_ms times  */
     uint64_t tasks;           /* directed edges of the roots counted         */
     int32_t grid_blocks;      /* launched workgroups                         */
     int32_t block_threads;
     int32_t lds_bytes;        /* dynamic LDS per workgroup                   */
-    int32_t reserved;
+    int32_t split_tasks;      /* array store: subtasks handed to the split
+                                 launch                                      */
+    int32_t split_overflow;   /* of those, re-run by the HBM launch          */
 } gat_count_stats;
 
 #define GAT_FLAG_ACCOUNT 1u   /* instrumented pass: accumulate W_alg bytes  */
