@@ -133,10 +133,12 @@ typedef struct gat_kernel_config {
                                  replaces binary search in their lists
                                  (0 = 256, negative = none)                   */
     int32_t split_us;         /* array store: a task still running after this
-                                 many microseconds hands the rest of its
-                                 depth-2 loop to a split launch in up to 64
-                                 parts (0 = 2000, negative = never); counts
-                                 and W_alg do not change                      */
+                                 many microseconds, once its launch's queue
+                                 is drained, hands the unfinished rest of its
+                                 shallowest loop at depth >= 2 to subtask
+                                 launches in up to 64 parts (0 = 2000,
+                                 negative = never); counts and W_alg do not
+                                 change                                       */
 } gat_kernel_config;
 
 /* Per-call statistics filled by gat_count (any pointer may be NULL). */
