@@ -61,3 +61,46 @@ def test_shard_argument_checks(gat):
         assert e.value.code == GAT_EINVAL
     with pytest.raises(GatError):
         plan.count_info(1)
+
+
+class StubPlan:
+    """Stands in for a Plan: shard raw sums are sums over roots of known
+    per-root values, and each shard 'takes' time proportional to its work."""
+
+    class G:
+        def __init__(self, n):
+            self.n = n
+
+        def v_cnt(self):
+            return self.n
+
+    def __init__(self, per_root, divisor=2, multiplier=3):
+        self.per_root = per_root
+        self.graph = self.G(len(per_root))
+        self.patterns = ["p"]
+        self.divisor, self.multiplier = divisor, multiplier
+        self.calls = []
+
+    def count_shard(self, lo, hi, config=None):
+        self.calls.append((lo, hi))
+        work = sum(self.per_root[lo:hi])
+        return [work], {"seconds": 1e-3 * (hi - lo) + 1e-6 * work, "split_tasks": 0}
+
+    def count_info(self, i):
+        return {"divisor": self.divisor, "multiplier": self.multiplier, "relabeled": True, "restrictions": []}
+
+
+def test_sharded_driver_partitions_roots_and_divides_once():
+    from graphautotune_amd.sharded import count_sharded, raw_sharded
+    per_root = [4000] * 5 + [2] * 3000 + [0] * 17
+    plan = StubPlan(per_root)
+    counts, shards = count_sharded(plan, target_s=0.05)
+    assert counts == [sum(per_root) // 2 * 3]
+    # the shards partition [0, n) in order and adapt their size
+    assert shards[0][0] == 0 and shards[-1][1] == len(per_root)
+    assert all(a[1] == b[0] for a, b in zip(shards, shards[1:]))
+    assert max(hi - lo for lo, hi, _ in shards) > 10 * (shards[0][1] - shards[0][0])
+    raw, sh = raw_sharded(plan, target_s=0.05, roots=(100, 2000))
+    assert raw == [sum(per_root[100:2000])] and sh[0][0] == 100 and sh[-1][1] == 2000
+    with pytest.raises(RuntimeError):
+        count_sharded(StubPlan([1, 2, 4]), target_s=1.0)  # raw 7 is not a multiple of 2
