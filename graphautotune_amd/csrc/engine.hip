@@ -33,6 +33,7 @@
 namespace gatk {
 
 constexpr int W = 64;
+constexpr int kSplitLevels = 3;  // LDS subtask levels (see KArgs)
 
 // A split subtask: candidates [lo, hi) of the loop of node `node` (at depth
 // `depth` >= 2) below root edge `edge` (directed edge index: v0 = its row,
@@ -77,11 +78,12 @@ struct KArgs {
     // clock while the queue it came from is drained hands the rest of its
     // shallowest unfinished loop at depth >= 2 to subs[out_base ..] (count in
     // *out_n, at most out_cap used). The main launch makes level-1 subtasks;
-    // the level-1 subtask launch (sub_mode 1) runs them and may split once
-    // more; the level-2 launch (sub_mode 2) runs those without splitting.
-    // Subtasks that outgrow LDS are copied to the third list
+    // the launch of level k (sub_mode k, LDS buffers) runs them and may split
+    // into level k + 1, up to level kSplitLevels, which does not split.
+    // Subtasks that outgrow LDS are copied to the HBM list
     // (subs[sovf_base ..], count *sovf_n), which also takes the splits of the
-    // HBM launch; a last subtask launch with HBM buffers (sub_mode 3) runs it.
+    // HBM launch; a last subtask launch with HBM buffers (sub_mode
+    // kSplitLevels + 1) runs it.
     int sub_mode;
     SplitTask *subs;
     unsigned long long *in_n, *out_n, *sovf_n;
@@ -445,7 +447,7 @@ __device__ __noinline__ void cancel_donations(const KArgs &a, WaveState *S, int 
 // came from has nothing left for idle waves.
 template <bool GS>
 __device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
-    if (a.split_ticks <= 0 || a.sub_mode >= 2) return false;
+    if (a.split_ticks <= 0 || a.sub_mode >= kSplitLevels) return false;
     if (uni64(wall_clock64()) - S->t0 <= a.split_ticks) return false;
     if (GS) {  // the HBM launch: its overflow list is drained
         const unsigned long long made = __atomic_load_n(a.ovf_n, __ATOMIC_RELAXED);
@@ -456,17 +458,19 @@ __device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
         return (long long)__atomic_load_n(&a.queue[0], __ATOMIC_RELAXED) >= a.e_hi - a.e_lo;
     const unsigned long long made = __atomic_load_n(a.in_n, __ATOMIC_RELAXED);
     const unsigned long long total = made < (unsigned long long)a.in_cap ? made : (unsigned long long)a.in_cap;
-    return __atomic_load_n(&a.queue[2], __ATOMIC_RELAXED) >= total;
+    return __atomic_load_n(&a.queue[1 + a.sub_mode], __ATOMIC_RELAXED) >= total;
 }
 
 // Hands candidates [lo, hi) of the depth-d frame's node to the next split
 // launch, in at most 64 chunks of at least min_chunk. False, with nothing
 // handed on, when the wave's record of donations or the list is full
-// (reserved entries past the end of the list are left empty).
-__device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int hi, int min_chunk) {
+// (reserved entries past the end of the list are left empty). Donations are
+// recorded for cancellation only where the donor can still abort (LDS
+// buffers); the HBM launch's tasks cannot, so their donations are unlimited.
+__device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int hi, int min_chunk, bool record) {
     const KArgs &a = C.a;
     WaveState *S = C.S;
-    if (S->n_don >= 8) return false;
+    if (record && S->n_don >= 8) return false;
     const long long edge = S->sub_edge >= 0 ? S->sub_edge : S->rb + (S->f_cur[1] - 1);
     const int r = hi - lo;
     const int chunk = max(min_chunk, (r + W - 1) / W);
@@ -486,7 +490,7 @@ __device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int h
         for (int j = 2; j < d; j++) t->pos[j] = j < S->base_depth ? S->path_pos[j] : S->f_cur[j] - 1;
     }
     if (fit < parts) return false;
-    if (C.ln == 0) {
+    if (record && C.ln == 0) {
         S->don_base[S->n_don] = a.out_base + base;
         S->don_cnt[S->n_don] = parts;
         S->n_don++;
@@ -506,7 +510,7 @@ __device__ __noinline__ bool split_shallowest(const Ctx &C, int d) {
     for (int k = uni(S->base_depth); k <= d; k++) {
         const int cur = uni(S->f_cur[k]), fh = uni(S->f_hi[k]);
         if (fh - cur < 2) continue;
-        if (!donate(C, k, uni(S->f_node[k]), cur, fh, C.P.nodes[uni(S->f_node[k])].leaf ? W : 1)) return false;
+        if (!donate(C, k, uni(S->f_node[k]), cur, fh, C.P.nodes[uni(S->f_node[k])].leaf ? W : 1, !GS)) return false;
         if (C.ln == 0) S->f_hi[k] = cur;
         wsync();
         return true;
@@ -694,7 +698,8 @@ __device__ __forceinline__ void leaf(const Ctx &C, const GatNode &N, int d, int 
             if (ln == 0) S->task_acc[Tm.pattern] += v;
         }
         wsync();
-        if (can_split && hi - (cb + W) >= 2 * W && want_split<GS>(a, S) && donate(C, d, node, cb + W, hi, W)) break;
+        if (can_split && hi - (cb + W) >= 2 * W && want_split<GS>(a, S) && donate(C, d, node, cb + W, hi, W, !GS))
+            break;
     }
     if (a.account) {
         bytes = wave_sum64(bytes);
@@ -872,7 +877,7 @@ __device__ __forceinline__ void run_sub(const Ctx &C, const SplitTask *g) {
     if (C.ln == 0) S->ebytes = eb;
     wsync();
     if (!ok) return;
-    const bool can_split = !GS && a.sub_mode == 1;
+    const bool can_split = !GS && a.sub_mode < kSplitLevels;
     push_frame<GS>(C, D, sub_node, sub_lo, sub_hi);
     int d = D;
     while (d >= D) {
@@ -977,8 +982,9 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
     if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
 }
 
-// Subtask launches: list sub_mode (1, 2: LDS buffers; 3: HBM buffers). A
-// subtask that outgrows LDS is copied to the third list for the last launch.
+// Subtask launches: list sub_mode (1 .. kSplitLevels: LDS buffers; then the
+// HBM list with HBM buffers). A subtask that outgrows LDS is copied to the
+// HBM list for the last launch.
 template <bool GS>
 __global__ void __launch_bounds__(256) engine_sub_kernel(KArgs a, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1654,7 +1660,10 @@ static int ensure_hubs(const Graph &g, DeviceGraph *dg, int thr) {
 
 // Split subtasks: capacity of each level's list, and the default time a
 // task runs in the tail of a launch before it hands on its unfinished loops.
+// Subtask counters live at ovf_n[8 + k] (k = 1 .. levels: LDS levels, then
+// the HBM list).
 constexpr long long kSubCap = 1ll << 20;
+constexpr int kLevels = gatk::kSplitLevels;
 constexpr int kSplitUs = 2000;
 
 struct Launch {
@@ -1720,8 +1729,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         HIPCHECK(hipMalloc(&d->plan, sizeof(GatPlanDev)));
         HIPCHECK(hipMemcpy(d->plan, &pd, sizeof(GatPlanDev), hipMemcpyHostToDevice));
         HIPCHECK(hipMalloc(&d->counts, sizeof(unsigned long long) * GATP_MAX_TERMS));
-        HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 8));
-        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 8));
+        HIPCHECK(hipMalloc(&d->queue, sizeof(unsigned long long) * 16));
+        HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 16));
         HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 4));
         HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
         for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
@@ -1747,9 +1756,9 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     // overflow entries: edges or tasks, and split subtasks
     const long long ovf_cap = clique ? 1 : (edge_grain ? g.m + 1 : (g.m + chunk - 1) / chunk + 1);
     const bool split = !clique && cfg.split_us > 0;
-    // subtask lists: level 1 and level 2 (kSubCap each), then the third list
-    // (LDS overflow of levels 1-2 plus the HBM launch's splits, 3 kSubCap)
-    if (split && !d->subs) HIPCHECK(hipMalloc(&d->subs, sizeof(gatk::SplitTask) * 5 * kSubCap));
+    // subtask lists: the LDS levels (kSubCap each), then the HBM list (LDS
+    // overflow of the levels plus the HBM launch's splits, 3 kSubCap)
+    if (split && !d->subs) HIPCHECK(hipMalloc(&d->subs, sizeof(gatk::SplitTask) * (kLevels + 3) * kSubCap));
     if (ovf_cap > d->ovf_cap) {
         if (d->ovf) HIPCHECK(hipFree(d->ovf));
         for (auto &t : d->tier)
@@ -1792,11 +1801,11 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     a.sub_mode = 0;
     a.subs = d->subs;
     a.in_n = nullptr;
-    a.out_n = d->ovf_n + 1;  // the main launch splits into level 1
+    a.out_n = d->ovf_n + 8 + 1;  // the main launch splits into level 1
     a.in_base = a.out_base = 0;
     a.in_cap = a.out_cap = kSubCap;
-    a.sovf_n = d->ovf_n + 4;
-    a.sovf_base = 2 * kSubCap;
+    a.sovf_n = d->ovf_n + 8 + kLevels + 1;
+    a.sovf_base = kLevels * kSubCap;
     a.sovf_cap = 3 * kSubCap;
     a.split_ticks = split ? (long long)cfg.split_us * di.wall_khz / 1000 : 0;
     if (!clique && cfg.hub_degree > 0) {
@@ -1889,8 +1898,8 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
 static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     hipStream_t s = d->stream;
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->counts, (long long)GATP_MAX_TERMS, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 8ll, 0ull);
-    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 8ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 16ll, 0ull);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 16ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->bytes, 4ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
     gatk::KArgs a = L.a, ao = L.ao;
@@ -1920,33 +1929,29 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
         hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1>), L.big_grid,
                            dim3(64 * gatk::CBLOCKW1), 0, s, c);
     } else {
-        if (a.split_ticks > 0) {  // split subtasks, in LDS like the main launch
-            gatk::KArgs s1 = a, s2 = a;
-            s1.sub_mode = 1;  // level 1: may split once more, into level 2
-            s1.in_n = d->ovf_n + 1;
-            s1.in_base = 0;
-            s1.out_n = d->ovf_n + 3;
-            s1.out_base = kSubCap;
-            s1.bytes = s2.bytes = d->bytes + 1;
-            s2.sub_mode = 2;  // level 2: no further splitting
-            s2.in_n = d->ovf_n + 3;
-            s2.in_base = kSubCap;
-            s2.out_n = nullptr;
-            hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, s1, L.wave_bytes);
-            hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, s2, L.wave_bytes);
-        }
-        ao.out_n = d->ovf_n + 4;  // the HBM launch splits into the third list
-        ao.out_base = 2 * kSubCap;
+        if (a.split_ticks > 0)  // split subtasks, level by level, in LDS like the main launch
+            for (int k = 1; k <= kLevels; k++) {
+                gatk::KArgs sk = a;
+                sk.sub_mode = k;
+                sk.in_n = d->ovf_n + 8 + k;
+                sk.in_base = (k - 1) * kSubCap;
+                sk.out_n = k < kLevels ? d->ovf_n + 8 + k + 1 : nullptr;  // the last level does not split
+                sk.out_base = k * kSubCap;
+                sk.bytes = d->bytes + 1;
+                hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, sk, L.wave_bytes);
+            }
+        ao.out_n = d->ovf_n + 8 + kLevels + 1;  // the HBM launch splits into the HBM list
+        ao.out_base = kLevels * kSubCap;
         ao.out_cap = 3 * kSubCap;
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
         if (ao.split_ticks > 0) {
-            gatk::KArgs s3 = ao;
-            s3.sub_mode = 3;  // HBM buffers, no further splitting
-            s3.in_n = d->ovf_n + 4;
-            s3.in_base = 2 * kSubCap;
-            s3.in_cap = 3 * kSubCap;
-            s3.out_n = nullptr;
-            hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, s3, L.ovf_wave_bytes);
+            gatk::KArgs sh = ao;
+            sh.sub_mode = kLevels + 1;  // HBM buffers, no further splitting
+            sh.in_n = d->ovf_n + 8 + kLevels + 1;
+            sh.in_base = kLevels * kSubCap;
+            sh.in_cap = 3 * kSubCap;
+            sh.out_n = nullptr;
+            hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, sh, L.ovf_wave_bytes);
         }
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
     }
@@ -1961,7 +1966,7 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
                   bool raw_sums = false) {
     PlanDevice *d = p->dev;
     const int np = p->cp.dev.n_patterns;
-    unsigned long long raw[GATP_MAX_TERMS], ovf[8] = {}, bytes[3] = {0, 0, 0};
+    unsigned long long raw[GATP_MAX_TERMS], ovf[16] = {}, bytes[3] = {0, 0, 0};
     unsigned int err = 0;
     HIPCHECK(hipMemcpyAsync(raw, d->counts, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(ovf, d->ovf_n, sizeof(ovf), hipMemcpyDeviceToHost, d->stream));
@@ -1990,11 +1995,10 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
             st->second_kernel_ms = ms_second;
         }
         st->overflow_tasks = L.clique ? ovf[3] : ovf[0];
-        st->split_tasks = L.clique ? 0
-                                   : (int32_t)(std::min<unsigned long long>(ovf[1], (unsigned long long)kSubCap) +
-                                               std::min<unsigned long long>(ovf[3], (unsigned long long)kSubCap) +
-                                               std::min<unsigned long long>(ovf[4], (unsigned long long)(3 * kSubCap)) -
-                                               ovf[2]);
+        unsigned long long made = 0;
+        for (int k = 1; k <= kLevels; k++) made += std::min<unsigned long long>(ovf[8 + k], (unsigned long long)kSubCap);
+        made += std::min<unsigned long long>(ovf[8 + kLevels + 1], (unsigned long long)(3 * kSubCap));
+        st->split_tasks = L.clique ? 0 : (int32_t)(made - ovf[2]);
         st->split_overflow = L.clique ? 0 : (int32_t)ovf[2];
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
