@@ -1,11 +1,11 @@
 """Split subtasks of the array store (GPU, through the C ABI).
 
-A main-launch task that has run longer than split_us hands the rest of its
-depth-2 loop to a split launch; with split_us = 1 nearly every task does.
-Counts must equal the oracle and the unsplit run, W_alg must not change
-(rebuilt sets are not charged twice), and a split subtask that outgrows LDS
-must be finished by the HBM launch, also under shards, reference mode and
-hipGraph replay.
+A task that has run longer than split_us once its launch's queue is drained
+hands the unfinished rest of its shallowest loop at depth >= 2 to subtask
+launches; with split_us = 1 nearly every task does. Counts must equal the
+oracle and the unsplit run, W_alg must not change (rebuilt sets are not
+charged twice), and a subtask that outgrows LDS must be finished with HBM
+buffers, also under shards, reference mode and hipGraph replay.
 """
 import sys
 
