@@ -29,6 +29,8 @@ def lib():
             f.restype = C.c_uint64
         L.orc_reference_count.argtypes = [C.c_int32, P, P, P, C.POINTER(C.c_int)]
         L.orc_reference_count.restype = C.c_uint64
+        L.orc_reference_count_roots.argtypes = [C.c_int32, P, P, P, C.c_int32, C.c_int32, C.POINTER(C.c_int)]
+        L.orc_reference_count_roots.restype = C.c_uint64
         L.orc_clique_walg.argtypes = [C.c_int32, P, P, C.c_int, C.POINTER(C.c_uint64)]
         L.orc_clique_walg.restype = C.c_uint64
         L.orc_clique_walg_sample.argtypes = [C.c_int32, P, P, C.c_int, C.c_int32, C.c_int32,
@@ -111,6 +113,15 @@ def restricted_count(off, adj, rows, restrictions, root_lo=0, root_hi=None):
     hi = n if root_hi is None else root_hi
     return int(lib().orc_restricted_count(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, k, r, nr,
                                           fa, sb, int(root_lo), int(hi)))
+
+
+def reference_count_roots(off, adj, sched_struct, root_lo, root_hi):
+    """reference_count over first vertices v0 in [root_lo, root_hi)."""
+    off, adj, n = _csr(off, adj)
+    ok = C.c_int()
+    v = lib().orc_reference_count_roots(n, off.ctypes.data, adj.ctypes.data if len(adj) else None,
+                                        C.byref(sched_struct), int(root_lo), int(root_hi), C.byref(ok))
+    return int(v), bool(ok.value)
 
 
 def relabel(off, adj):

@@ -1018,6 +1018,11 @@ static void ref_level(ref_t *R, int l) {
 
 uint64_t orc_reference_count(int32_t n, const int64_t *off, const int32_t *adj, const gat_schedule_data *s,
                              int *ok) {
+    return orc_reference_count_roots(n, off, adj, s, 0, n, ok);
+}
+
+uint64_t orc_reference_count_roots(int32_t n, const int64_t *off, const int32_t *adj, const gat_schedule_data *s,
+                                   int32_t root_lo, int32_t root_hi, int *ok) {
     ref_t R;
     int p, l;
     int32_t maxd = 0, v;
@@ -1036,8 +1041,10 @@ uint64_t orc_reference_count(int32_t n, const int64_t *off, const int32_t *adj, 
     R.n = n;
     R.s = s;
     /* perform_search + first_extend_kernel (engine.cuh:232-265,
-     * engine_kernel.cuh:40-60) */
-    for (v = 0; v < n; v++) {
+     * engine_kernel.cuh:40-60): every first vertex starts its own walk */
+    if (root_lo < 0) root_lo = 0;
+    if (root_hi > n) root_hi = n;
+    for (v = root_lo; v < root_hi; v++) {
         R.chosen[0] = v;
         ref_build(&R, 0, v);
         ref_level(&R, 0);

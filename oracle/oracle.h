@@ -52,6 +52,9 @@ uint64_t orc_restricted_count(int32_t n, const int64_t *off, const int32_t *adj,
  * the schedule makes the reference read an undefined buffer. */
 uint64_t orc_reference_count(int32_t n, const int64_t *off, const int32_t *adj, const gat_schedule_data *s,
                              int *ok);
+/* The same restricted to first vertices v0 in [root_lo, root_hi). */
+uint64_t orc_reference_count_roots(int32_t n, const int64_t *off, const int32_t *adj, const gat_schedule_data *s,
+                                   int32_t root_lo, int32_t root_hi, int *ok);
 /* k-clique walk on a graph whose ids are already in descending-degree
  * order: returns #K_k and the algorithmic bytes in *walg. */
 uint64_t orc_clique_walg(int32_t n, const int64_t *off, const int32_t *adj, int k, uint64_t *walg);

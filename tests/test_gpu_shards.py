@@ -187,3 +187,23 @@ def test_count_sharded_equals_count(gat, gpu):
         got, shards = count_sharded(plan, target_s=1e-4, first=3)
         assert got == plan.count()[0], (pats, mode)
         assert len(shards) > 3 and shards[-1][1] == g.v_cnt()
+
+
+def test_reference_mode_shards_match_oracle(gat, orc, gpu):
+    """Reference mode (input ids, the reference's restrictions and IEP):
+    every shard equals the oracle's restatement of the reference engine over
+    the same first vertices."""
+    from graphautotune_amd._lib import ScheduleData
+    for g in (gat.Graph.rmat(8, 6, 0.57, 0.19, 0.19, 3), gat.Graph.chung_lu(300, 6.0, 2.1, 40, 5)):
+        off, adj = g.csr()
+        cuts = shard_cuts(g.v_cnt())
+        for p in (TRIANGLE, K4, HOUSE, C5):
+            s = orc.schedule(p, ScheduleData)
+            want = [orc.reference_count_roots(off, adj, s, lo, hi) for lo, hi in zip(cuts, cuts[1:])]
+            if not all(ok for _, ok in want):
+                continue
+            plan = gat.Plan(g, [p], "reference")
+            assert not plan.count_info(0)["relabeled"]
+            for cfg in ({}, OVERFLOW):
+                got = [plan.count_shard(lo, hi, cfg)[0][0] for lo, hi in zip(cuts, cuts[1:])]
+                assert got == [v for v, _ in want], (p, cfg)

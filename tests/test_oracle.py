@@ -339,3 +339,22 @@ def test_restricted_count_against_brute_force(orc, seed):
         for lo, hi in ((0, n), (0, 3), (3, 8), (2, 5), (7, 8), (4, 4)):
             want = brute_restricted(off, adj, rows, R, lo, hi)
             assert orc.restricted_count(off, adj, rows, R, lo, hi) == want, (pat, R, lo, hi)
+
+
+def test_reference_count_roots_partitions_the_count(orc):
+    """Every first vertex starts its own walk, so the root-range form adds up
+    over a partition to the full reference count (pinned itself by k * unique
+    for cliques and by the brute-force restatement above)."""
+    from graphautotune_amd._lib import ScheduleData
+    rng = np.random.default_rng(4)
+    n = 40
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.2]
+    off, adj = csr_of_edges(n, edges)
+    for p in (TRIANGLE, K4, HOUSE, C5, P3):
+        s = orc.schedule(p, ScheduleData)
+        full, ok = orc.reference_count(off, adj, s)
+        if not ok:
+            continue
+        cuts = [0, 1, 7, 20, 39, 40]
+        parts = [orc.reference_count_roots(off, adj, s, lo, hi) for lo, hi in zip(cuts, cuts[1:])]
+        assert all(o for _, o in parts) and sum(v for v, _ in parts) == full, p
