@@ -19,6 +19,7 @@ GAT_ENODEV = -4
 GAT_ELIMIT = -5
 GAT_EUNSUPPORTED = -6
 GAT_EINFEASIBLE = -7
+GAT_EINTERNAL = -8
 
 MODE_UNIQUE = 0
 MODE_REFERENCE = 1

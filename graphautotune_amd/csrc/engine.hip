@@ -467,7 +467,7 @@ __device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
 // (reserved entries past the end of the list are left empty). Donations are
 // recorded for cancellation only where the donor can still abort (LDS
 // buffers); the HBM launch's tasks cannot, so their donations are unlimited.
-__device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int hi, int min_chunk, bool record) {
+__device__ __noinline__ bool donate(const Ctx C, int d, int node, int lo, int hi, int min_chunk, bool record) {
     const KArgs &a = C.a;
     WaveState *S = C.S;
     if (record && S->n_don >= 8) return false;
@@ -505,7 +505,7 @@ __device__ __noinline__ bool donate(const Ctx &C, int d, int node, int lo, int h
 // candidate being walked at depth k, which later donations from deeper
 // frames record as their path.
 template <bool GS>
-__device__ __noinline__ bool split_shallowest(const Ctx &C, int d) {
+__device__ __noinline__ bool split_shallowest(const Ctx C, int d) {
     WaveState *S = C.S;
     for (int k = uni(S->base_depth); k <= d; k++) {
         const int cur = uni(S->f_cur[k]), fh = uni(S->f_hi[k]);
@@ -913,9 +913,18 @@ __device__ __forceinline__ void run_sub(const Ctx &C, const SplitTask *g) {
     }
 }
 
+// The array-store kernels read their arguments from device memory (written
+// by put_kargs just before the launch): the walk hands a reference to them
+// to out-of-line helpers, and a reference to a by-value kernel argument
+// would make every lane copy the whole record to scratch first.
+__global__ void put_kargs(KArgs a, KArgs *dst) {
+    if (threadIdx.x == 0) *dst = a;
+}
+
 template <bool GS>
-__global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
+__global__ void __launch_bounds__(256) engine_kernel(const KArgs *__restrict__ ap, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const KArgs &a = *ap;
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
     const int wpb = blockDim.x / W;
     WaveState *S = reinterpret_cast<WaveState *>(smem + (size_t)wid * wave_bytes);
@@ -986,8 +995,9 @@ __global__ void __launch_bounds__(256) engine_kernel(KArgs a, int wave_bytes) {
 // HBM list with HBM buffers). A subtask that outgrows LDS is copied to the
 // HBM list for the last launch.
 template <bool GS>
-__global__ void __launch_bounds__(256) engine_sub_kernel(KArgs a, int wave_bytes) {
+__global__ void __launch_bounds__(256) engine_sub_kernel(const KArgs *__restrict__ ap, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
+    const KArgs &a = *ap;
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
     const int wpb = blockDim.x / W;
     WaveState *S = reinterpret_cast<WaveState *>(smem + (size_t)wid * wave_bytes);
@@ -1559,6 +1569,7 @@ struct PlanDevice {
     unsigned int *err = nullptr;
     long long *ovf = nullptr;
     gatk::SplitTask *subs = nullptr;  // split subtasks (array store)
+    gatk::KArgs *dargs = nullptr;     // arguments of the array-store launches of a count
     long long *tier[3] = {};  // clique vertices of the workgroup tiers
     long long ovf_cap = 0;
     int *gstore = nullptr;
@@ -1581,7 +1592,7 @@ void destroy_plan_device(PlanDevice *d) {
     if (d->graph) (void)hipGraphDestroy(d->graph);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
-    for (void *p : {(void *)d->plan, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
+    for (void *p : {(void *)d->plan, (void *)d->dargs, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
                     (void *)d->err, (void *)d->ovf, (void *)d->subs, (void *)d->tier[0], (void *)d->tier[1], (void *)d->tier[2],
                     (void *)d->gstore, (void *)d->grows,
                     (void *)d->goff})
@@ -1665,6 +1676,9 @@ static int ensure_hubs(const Graph &g, DeviceGraph *dg, int thr) {
 constexpr long long kSubCap = 1ll << 20;
 constexpr int kLevels = gatk::kSplitLevels;
 constexpr int kSplitUs = 2000;
+// device argument records of one array-store count: the main launch, the
+// LDS subtask levels, the HBM launch and the HBM subtask launch
+constexpr int kArgSlots = kLevels + 3;
 
 struct Launch {
     gatk::KArgs a;
@@ -1733,6 +1747,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         HIPCHECK(hipMalloc(&d->ovf_n, sizeof(unsigned long long) * 16));
         HIPCHECK(hipMalloc(&d->bytes, sizeof(unsigned long long) * 4));
         HIPCHECK(hipMalloc(&d->err, sizeof(unsigned int) * 2));
+        HIPCHECK(hipMalloc(&d->dargs, sizeof(gatk::KArgs) * kArgSlots));
         for (auto &e : d->ev) HIPCHECK(hipEventCreate(&e));
     }
     PlanDevice *d = p->dev;
@@ -1911,7 +1926,8 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     if (L.clique) {
         hipLaunchKernelGGL(gatk::clique_kernel, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
     } else {
-        hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, a, L.wave_bytes);
+        hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, a, d->dargs);
+        hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, d->dargs, L.wave_bytes);
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
@@ -1938,12 +1954,16 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
                 sk.out_n = k < kLevels ? d->ovf_n + 8 + k + 1 : nullptr;  // the last level does not split
                 sk.out_base = k * kSubCap;
                 sk.bytes = d->bytes + 1;
-                hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, sk, L.wave_bytes);
+                hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, sk, d->dargs + k);
+                hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, d->dargs + k,
+                                   L.wave_bytes);
             }
         ao.out_n = d->ovf_n + 8 + kLevels + 1;  // the HBM launch splits into the HBM list
         ao.out_base = kLevels * kSubCap;
         ao.out_cap = 3 * kSubCap;
-        hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, ao, L.ovf_wave_bytes);
+        hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, ao, d->dargs + kLevels + 1);
+        hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, d->dargs + kLevels + 1,
+                           L.ovf_wave_bytes);
         if (ao.split_ticks > 0) {
             gatk::KArgs sh = ao;
             sh.sub_mode = kLevels + 1;  // HBM buffers, no further splitting
@@ -1951,7 +1971,9 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
             sh.in_base = kLevels * kSubCap;
             sh.in_cap = 3 * kSubCap;
             sh.out_n = nullptr;
-            hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, sh, L.ovf_wave_bytes);
+            hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, sh, d->dargs + kLevels + 2);
+            hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s,
+                               d->dargs + kLevels + 2, L.ovf_wave_bytes);
         }
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
     }
@@ -1973,7 +1995,10 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
     HIPCHECK(hipMemcpyAsync(bytes, d->bytes, sizeof(bytes), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipMemcpyAsync(&err, d->err, sizeof(err), hipMemcpyDeviceToHost, d->stream));
     HIPCHECK(hipStreamSynchronize(d->stream));
-    if (err) GAT_FAIL(GAT_ENODEV, "engine reported internal error flags 0x%x", err);
+    // flags 2 and 4: a device work list (overflow tasks, subtasks, clique
+    // tiers) filled up; 1 and 8: states the host's sizing rules out
+    if (err & 6u) GAT_FAIL(GAT_ELIMIT, "a device work list overflowed (engine flags 0x%x)", err);
+    if (err) GAT_FAIL(GAT_EINTERNAL, "engine reported an internal inconsistency (flags 0x%x)", err);
     for (int i = 0; i < np && raw_sums; i++)
         if (counts_out) counts_out[i] = raw[i];
     for (int i = 0; i < np && !raw_sums; i++) {

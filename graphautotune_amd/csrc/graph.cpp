@@ -95,6 +95,25 @@ int check_csr(const Graph &g) {
     }
     if (bad_v >= 0)
         GAT_FAIL(GAT_EINVAL, "row %d is not sorted/unique/loop-free or holds an out-of-range id", bad_v);
+    // every entry needs its reverse: the engine's orientation (split[]),
+    // hub rows and root shards all assume symmetric adjacency
+    std::atomic<int64_t> asym{-1};
+    parallel_for(g.n, [&](int64_t lo, int64_t hi) {
+        for (int64_t v = lo; v < hi && asym.load(std::memory_order_relaxed) < 0; v++)
+            for (int64_t e = g.off[v]; e < g.off[v + 1]; e++) {
+                const int32_t w = g.adj[e];
+                auto b = g.adj.begin() + g.off[w], en = g.adj.begin() + g.off[w + 1];
+                if (!std::binary_search(b, en, int32_t(v))) {
+                    asym.store(e, std::memory_order_relaxed);
+                    break;
+                }
+            }
+    });
+    const int64_t e = asym.load();
+    if (e >= 0) {
+        const int32_t v = int32_t(std::upper_bound(g.off.begin(), g.off.end(), e) - g.off.begin() - 1);
+        GAT_FAIL(GAT_EINVAL, "CSR is not symmetric: entry (%d,%d) has no reverse", v, g.adj[e]);
+    }
     return GAT_OK;
 }
 

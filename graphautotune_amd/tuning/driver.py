@@ -5,13 +5,19 @@ result files; here the engine runs in-process over the C ABI).
 Every run's counts must equal the first run's: a config that changes a
 count is a bug, not a fast config. An infeasible config (e.g. too much LDS)
 returns FLOAT_INF, as the reference maps a failing run (driver.py:118-122).
+An engine inconsistency (GAT_EINTERNAL) is raised as EngineBug, never
+scored: it says nothing about the config's speed.
 """
 import statistics
 
 from .. import core
-from .._lib import GAT_EINFEASIBLE, GatError
+from .._lib import GAT_EINFEASIBLE, GAT_EINTERNAL, GatError
 
 FLOAT_INF = 1e38  # tuning/tuning/common/const.py:3
+
+
+class EngineBug(RuntimeError):
+    """The engine reported an internal inconsistency for a config."""
 
 
 class Driver:
@@ -40,6 +46,8 @@ class Driver:
             if e.code == GAT_EINFEASIBLE:
                 self.log.append((cfg, FLOAT_INF))
                 return FLOAT_INF
+            if e.code == GAT_EINTERNAL:
+                raise EngineBug("engine bug under config %s: %s" % (cfg, e)) from e
             raise
         t = statistics.median(secs)
         self.log.append((cfg, t))

@@ -33,9 +33,12 @@ extern "C" {
 #define GAT_EIO (-2)         /* file could not be read or written          */
 #define GAT_ENOMEM (-3)      /* host or device allocation failed           */
 #define GAT_ENODEV (-4)      /* no usable GPU / HIP runtime error          */
-#define GAT_ELIMIT (-5)      /* a fixed capacity would be exceeded         */
+#define GAT_ELIMIT (-5)      /* a fixed capacity would be exceeded (also a
+                                device work list that filled up)           */
 #define GAT_EUNSUPPORTED (-6)/* the request has no defined result          */
 #define GAT_EINFEASIBLE (-7) /* kernel config cannot launch (LDS, block)   */
+#define GAT_EINTERNAL (-8)   /* the engine flagged an inconsistency of its own
+                                (a bug, never a property of the input)     */
 
 /* ---- capacities of the schedule record ------------------------------ */
 /* MAX_VERTEXES / MAX_PREFIXS of include/consts/general_consts.hpp:3-4 are

@@ -77,6 +77,22 @@ def test_bin_rejects_truncated(gat, tmp_path):
         gat.Graph.load(p)
 
 
+def test_bin_rejects_asymmetric(gat, tmp_path):
+    # a sorted, loop-free .bin whose entry (0,2) has no reverse (2,0): the
+    # engine's orientation and shards assume symmetric rows
+    from graphautotune_amd._lib import GatError
+    off = np.array([0, 2, 3, 4], dtype=np.int64)
+    adj = np.array([1, 2, 0, 1], dtype=np.int32)
+    p = tmp_path / "asym.bin"
+    p.write_bytes(np.int32(3).tobytes() + np.int64(4).tobytes() + off.tobytes() + adj.tobytes())
+    with pytest.raises(GatError, match="symmetric"):
+        gat.Graph.load(p)
+    # rows {1,2}, {0}, {0}: symmetric
+    adj2 =np.array([1, 2, 0, 0], dtype=np.int32)
+    p.write_bytes(np.int32(3).tobytes() + np.int64(4).tobytes() + off.tobytes() + adj2.tobytes())
+    assert gat.Graph.load(p).e_cnt() == 4
+
+
 def test_txt_loader_semantics(gat, tmp_path):
     # graph_backend.cuh:23-89: relabel by sorted id, drop self-loops,
     # symmetrise, dedup; declared count may exceed the ids seen.
