@@ -10,9 +10,10 @@ counts. `roofline` is for whichever of the main and the second launch takes
 longer: W_alg bytes of that launch (instrumented pass) / its hipEvent-timed
 duration, against 8 TB/s; `hbm_achieved` is the same launch's measured HBM
 bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) over
-that duration. `cpu_baseline` times the CPU oracle's restatement
-of the same algorithm on a bounded sample of top vertices and scales it by
-algorithmic bytes.
+that duration. `cpu_baseline` times the CPU oracle's whole-graph K4 + K5
+count on the host's cores (at most 16) and checks its counts against the
+GPU's; `naive_c1` times the reference's own CPU path (naive_pm, restated) on
+its triangle workload, one thread.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -64,38 +65,98 @@ def timed(plan, cfg, steps):
     return counts, secs, (mains, seconds_k), walls, st
 
 
-def cpu_baseline(g, total_walg, budget_s):
-    """The oracle's restatement of the bitmap clique store (single thread) on
-    every stride-th top vertex, K4 then K5 as the plain plans run them,
-    scaled to the whole workload by algorithmic bytes (same definition on
-    both sides)."""
+def host_cpu():
+    """CPU model and the host threads this process may use (at most 16: the
+    share of the host that goes with one GPU)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, avail, max(1, min(16, avail))
+
+
+def cpu_baseline(g, gpu_counts):
+    """The oracle's whole-graph K4 + K5 count (a CPU port of the bitmap clique
+    store, oracle/clique_mt.c) on the host's cores, timed, its counts checked
+    against the GPU's. The degree relabelling is done before the timed span,
+    as the GPU plan does it at plan creation."""
     from oracle import api as orc
     off, adj = g.csr()
-    t0 = time.perf_counter()
     ro, ra = orc.relabel(off, adj)
-    log("cpu: relabel %.1fs" % (time.perf_counter() - t0))
-    n = len(ro) - 1
-    stride = 1 << 16
-    while True:
-        t0 = time.perf_counter()
-        w = 0
-        for k in (4, 5):
-            w += orc.clique_bitmap_walg_sample(ro, ra, k, stride, 0)[1]
-        dt = time.perf_counter() - t0
-        log("cpu: stride %d took %.2fs (W %.3g B)" % (stride, dt, w))
-        if dt >= budget_s / 4 or stride == 1:
-            break
-        stride = max(1, int(stride / min(64.0, (budget_s / 2) / max(dt, 0.01))))
-    est = dt * total_walg / max(w, 1)
+    model, avail, threads = host_cpu()
+    t0 = time.perf_counter()
+    k3, k4, k5 = orc.clique_counts_mt(ro, ra, 5, threads)
+    dt = time.perf_counter() - t0
+    log("cpu: whole-graph K4 %d K5 %d on %d threads in %.2f s (%s)" % (k4, k5, threads, dt, model))
+    if [k4, k5] != list(gpu_counts):
+        raise AssertionError("CPU oracle counts %s differ from the GPU's %s" % ([k4, k5], gpu_counts))
     return {
-        "value": g.undirected_edges() / est,
+        "value": g.undirected_edges() / dt,
         "unit": "edges/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": "oracle bitmap-store clique count (K4, then K5) over top vertices v0 = 0, %d, 2*%d, ... "
-                  "(%d of %d vertices, %.1f s, %.3g of %.3g algorithmic bytes); time scaled by bytes"
-                  % (stride, stride, (n + stride - 1) // stride, n, dt, w, total_walg),
+        "sample": "whole graph: the oracle's K4 + K5 count over all %d top vertices (one pass for both), "
+                  "%.2f s on %d threads; counts equal the GPU's" % (g.v_cnt(), dt, threads),
+        "seconds": dt,
+        "counts": {"K4": k4, "K5": k5},
+        "cpu_model": model,
+        "host_threads_available": avail,
     }
+
+
+def naive_c1():
+    """The reference's CPU path (target/naive_pm.cu:10-44, restated with a
+    uint64 count) on its own workload: triangles of G(n = 10,000,
+    p = 16/9,999), seed 1, one thread. It counts labeled embeddings, 6 per
+    triangle."""
+    import graphautotune_amd as gat
+    from oracle import api as orc
+    g = gat.Graph.erdos_renyi(10000, 16 / 9999, 1)
+    off, adj = g.csr()
+    t0 = time.perf_counter()
+    labeled = orc.naive_count(off, adj, "011101110")
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gpu, st = gat.Plan(g, ["011101110"], "labeled").count()
+    log("naive_pm restated: %d labeled triangles in %.1f s; GPU labeled count %d" % (labeled, dt, gpu[0]))
+    if gpu[0] != labeled:
+        raise AssertionError("naive_pm restatement %d and GPU labeled count %d differ" % (labeled, gpu[0]))
+    model, _, _ = host_cpu()
+    return {"workload": "Erdos-Renyi G(10000, 16/9999), seed 1, %d edges; labeled triangle embeddings"
+                        % g.undirected_edges(),
+            "labeled_count": labeled, "triangles": labeled // 6, "seconds": dt, "cores": 1,
+            "value": g.undirected_edges() / dt, "unit": "edges/s", "cpu_model": model,
+            "gpu_device_ms": st["seconds"] * 1e3, "kind": "reference algorithm, restated"}
+
+
+def timeline_summary(tl, tpm, names):
+    """Per launch: units, span, busy fraction (sum of unit durations over
+    units x span), percentiles of unit end times, and the row-build share of
+    the clique store's cycles."""
+    import numpy as np
+    out = {}
+    for lid in sorted(set(int(x) for x in tl[:, 0])):
+        r = tl[tl[:, 0] == lid]
+        t0, t1 = r[:, 2] / tpm, r[:, 3] / tpm
+        span = t1.max() - t0.min()
+        busy = float((t1 - t0).sum() / (len(r) * span)) if span > 0 else 1.0
+        ends = np.sort(t1 - t0.min())
+        hist, edges = np.histogram(t1 - t0.min(), bins=10, range=(0, span if span > 0 else 1))
+        cyc = r[:, 4].sum() + r[:, 5].sum()
+        out[names.get(lid, str(lid))] = {
+            "units": int(len(r)), "span_ms": float(span), "busy_frac": busy,
+            "end_ms_p0_p50_p90_p99_p100": [float(np.percentile(ends, q)) for q in (0, 50, 90, 99, 100)],
+            "end_hist_10bins": hist.tolist(),
+            "items": int(r[:, 6].sum()), "pairs": int(r[:, 7].sum()),
+            "build_cycle_share": float(r[:, 4].sum() / cyc) if cyc else None,
+        }
+    return out
 
 
 IEP_PATTERNS = {"house": "0100110110010110110010100", "c5": "0100110100010100010110010"}
@@ -114,6 +175,16 @@ def make_graph(gat, name):
     raise SystemExit("unknown graph %r (rmatS or clN)" % name)
 
 
+class _Shard:
+    """A plan restricted to a root range, with Plan.count's signature."""
+
+    def __init__(self, plan, roots):
+        self.plan, self.roots = plan, roots
+
+    def count(self, cfg=None, account=False, timeline=False):
+        return self.plan.count_shard(self.roots[0], self.roots[1], cfg, account=account, timeline=timeline)
+
+
 def iep_line(gat, args):
     """Auxiliary line: one 5-vertex IEP pattern on the array store (default
     config, hub bitmaps on), unique mode. Roofline of the longer of the main
@@ -123,11 +194,25 @@ def iep_line(gat, args):
     E = g.undirected_edges()
     plan = gat.Plan(g, [IEP_PATTERNS[args.iep]], "unique")
     cfg = json.loads(args.config) if args.config else {}
+    roots = tuple(int(x) for x in args.roots.split(":")) if args.roots else None
+    if roots:  # a root shard: raw sums of the roots [lo, hi) of the degree-ordered graph
+        plan = _Shard(plan, roots)
+        off = g.csr()[0]
+        E = int(off[roots[1]] - off[roots[0]])  # directed edges leaving the shard's roots
+        desc += "; roots [%d, %d) of the degree-ordered graph (raw sums)" % roots
     for _ in range(args.warmup):
         plan.count(cfg)
     _, acc = plan.count(cfg, account=True)
     counts, secs, (mains, seconds), walls, st = timed(plan, cfg, args.steps)
     t = statistics.median(secs)
+    if args.timeline:
+        _, tst = plan.count(cfg, timeline=True)
+        names = {0: "engine_kernel", 1: "subtasks level 1", 2: "subtasks level 2", 3: "subtasks level 3",
+                 4: "HBM launch", 5: "HBM subtasks"}
+        with open(args.timeline, "w") as f:
+            json.dump({"workload": desc, "pattern": args.iep, "kernel_config": cfg, "device_ms": tst["seconds"] * 1e3,
+                       "launches": timeline_summary(tst["timeline"], tst["ticks_per_ms"], names)}, f, indent=1)
+        log("timeline summary written to %s" % args.timeline)
     launches = [("gatk::engine_kernel<false>", statistics.median(mains), acc["alg_bytes_main"]),
                 ("split and HBM launches", statistics.median(seconds), acc["alg_bytes_second"])]
     name, ms, walg = max(launches, key=lambda x: x[1])
@@ -140,7 +225,8 @@ def iep_line(gat, args):
         if tr.get("config") == cfg:
             traffic = tr.get("bytes_per_launch", {}).get(name)
     print(json.dumps({
-        "metric": "exact-count throughput (edges/s), 5-vertex IEP pattern, array store",
+        "metric": "exact-count throughput (edges/s), 5-vertex IEP pattern, array store" +
+                  ("; edges = directed edges of the shard's roots" if roots else ""),
         "value": E / t, "unit": "edges/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sum(walls) / len(walls) * 1e3, "higher_is_better": True, "dtype": "int32",
         "data": "synthetic",
@@ -162,8 +248,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--timeline", default=None, metavar="PATH",
+                    help="also run one batched count with per-wave/per-workgroup timeline records and write "
+                         "their summary (load-imbalance histograms) to PATH as JSON")
     ap.add_argument("--config", type=str, default=None, help="JSON kernel config for the batched run")
     ap.add_argument("--no-plain", action="store_true",
                     help="time only the batched plan (profiling runs: one kernel config per kernel name)")
@@ -173,6 +261,8 @@ def main():
     ap.add_argument("--iep", choices=sorted(IEP_PATTERNS), default=None,
                     help="auxiliary line: one 5-vertex IEP pattern on the array store instead of K4 + K5")
     ap.add_argument("--graph", default="rmat14", help="with --iep: rmatS or clN (see make_graph)")
+    ap.add_argument("--roots", default=None, metavar="LO:HI",
+                    help="with --iep: count only the roots [LO, HI) of the degree-ordered graph (a shard)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -240,7 +330,7 @@ def main():
                  walg_second)]
     kernel_name, kernel_ms, kernel_walg = max(launches, key=lambda x: x[1])
     achieved = kernel_walg / (kernel_ms * 1e-3) / 1e9
-    t_plain, plain_ms = None, None
+    t_plain, plain_ms, t_plain_same, plain_same_ms = None, None, None, None
     if plains:
         c4, s4, _, _, _ = timed(plain4, {}, args.steps)
         c5, s5, _, _, _ = timed(plain5, {}, args.steps)
@@ -248,6 +338,16 @@ def main():
             raise AssertionError("batched %s and plain %s counts differ" % (counts, c4 + c5))
         plain_ms = {"K4": statistics.median(s4) * 1e3, "K5": statistics.median(s5) * 1e3}
         t_plain = statistics.median(s4) + statistics.median(s5)
+        # the same single-pattern plans under the batched run's (tuned)
+        # config: separates the gain of batching from that of tuning
+        for p in plains:
+            p.count(cfg)
+        c4, s4, _, _, _ = timed(plain4, cfg, args.steps)
+        c5, s5, _, _, _ = timed(plain5, cfg, args.steps)
+        if counts != [c4[0], c5[0]]:
+            raise AssertionError("batched %s and plain (tuned config) %s counts differ" % (counts, c4 + c5))
+        plain_same_ms = {"K4": statistics.median(s4) * 1e3, "K5": statistics.median(s5) * 1e3}
+        t_plain_same = statistics.median(s4) + statistics.median(s5)
     log("counts K4 %d K5 %d | batched %.3f ms (main %.3f ms, second %.3f ms), plain %s ms | overflow tasks %d"
         % (counts[0], counts[1], t_b * 1e3, main_ms, second_ms, "%.3f" % (t_plain * 1e3) if t_plain else "-",
            st["overflow_tasks"]))
@@ -288,6 +388,8 @@ def main():
         "device_ms_plain": plain_ms,
         "plain_value": E / t_plain if t_plain else None,
         "batched_vs_plain": t_plain / t_b if t_plain else None,
+        "device_ms_plain_same_config": plain_same_ms,
+        "batched_vs_plain_same_config": t_plain_same / t_b if t_plain_same else None,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -308,10 +410,21 @@ def main():
             "step_achieved": walg / t_b / 1e9,
         },
     }
+    if args.timeline:
+        _, tst = batched.count(cfg, timeline=True)
+        names = {0: "clique_kernel", 1: "clique_block_kernel tier 0", 2: "clique_block_kernel tier 1",
+                 3: "clique_block_kernel tier 2"} if bitmap else \
+                {0: "engine_kernel", 1: "subtasks level 1", 2: "subtasks level 2", 3: "subtasks level 3",
+                 4: "HBM launch", 5: "HBM subtasks"}
+        summ = {"workload": out["config"]["workload"], "kernel_config": cfg, "device_ms": tst["seconds"] * 1e3,
+                "launches_ms": {"main": tst["main_kernel_ms"], "second": tst["second_kernel_ms"]},
+                "launches": timeline_summary(tst["timeline"], tst["ticks_per_ms"], names)}
+        with open(args.timeline, "w") as f:
+            json.dump(summ, f, indent=1)
+        log("timeline summary written to %s" % args.timeline)
     if not args.no_cpu:
-        if plain_walg is None:
-            plain_walg = sum(gat.Plan(g, [p], "unique").count({}, account=True)[1]["alg_bytes"] for p in (K4, K5))
-        out["cpu_baseline"] = cpu_baseline(g, plain_walg, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(g, counts)
+        out["naive_c1"] = naive_c1()
     print(json.dumps(out), flush=True)
 
 

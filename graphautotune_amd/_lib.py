@@ -26,6 +26,7 @@ MODE_REFERENCE = 1
 MODE_LABELED = 2
 
 FLAG_ACCOUNT = 1
+FLAG_TIMELINE = 2
 
 MAXV = 10
 MAXP = 16
@@ -112,6 +113,7 @@ class CountStats(C.Structure):
         ("lds_bytes", C.c_int32),
         ("split_tasks", C.c_int32),
         ("split_overflow", C.c_int32),
+        ("split_level", C.c_int64 * 4),
     ]
 
 
@@ -181,6 +183,7 @@ _SIGS = {
     "gat_count_shard": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_uint32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "gat_plan_free": (None, [_P]),
+    "gat_plan_timeline": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "gat_plan_describe": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "gat_default_config": (C.c_int, [C.POINTER(KernelConfig)]),
     "gat_device_count": (C.c_int, [C.POINTER(C.c_int32)]),

@@ -196,13 +196,21 @@ class GlobalMemoryGraph:
         return self._info()[2]
 
     def csr(self):
-        """(offsets, adj) as numpy views into the library's host copy."""
+        """(offsets, adj) as read-only numpy views into the library's host
+        copy; the views keep this graph alive."""
         n, m, _ = self._info()
         po, pa = C.c_void_p(), C.c_void_p()
         check(lib().gat_graph_csr(self._h, C.byref(po), C.byref(pa)))
-        off = np.ctypeslib.as_array((C.c_int64 * (n + 1)).from_address(po.value))
-        adj = (np.ctypeslib.as_array((C.c_int32 * m).from_address(pa.value)) if m
-               else np.zeros(0, dtype=np.int32))
+
+        def view(ctype, count, addr):
+            buf = (ctype * count).from_address(addr)
+            buf._graph = self  # the array's base holds the graph, so its memory outlives every view
+            a = np.ctypeslib.as_array(buf)
+            a.flags.writeable = False
+            return a
+
+        off = view(C.c_int64, n + 1, po.value)
+        adj = view(C.c_int32, m, pa.value) if m else np.zeros(0, dtype=np.int32)
         return off, adj
 
     def triangles(self):
@@ -224,6 +232,14 @@ Graph = GlobalMemoryGraph
 
 
 SCHEDULES = {"reference": SCHEDULE_REFERENCE, "graph": SCHEDULE_GRAPH, "ranked": SCHEDULE_RANKED}
+
+
+def _stats_dict(st):
+    out = {}
+    for k, _ in CountStats._fields_:
+        v = getattr(st, k)
+        out[k] = list(v) if hasattr(v, "_length_") else v
+    return out
 
 
 class Plan:
@@ -276,8 +292,11 @@ class Plan:
             lib().gat_plan_describe(self._h, buf, len(buf))
         return buf.value.decode()
 
-    def count(self, config=None, account=False, graph_replays=0):
-        """Counts (list of int, one per pattern) and a stats dict."""
+    def count(self, config=None, account=False, graph_replays=0, timeline=False):
+        """Counts (list of int, one per pattern) and a stats dict. With
+        timeline, the stats hold "timeline": the per-wave / per-workgroup
+        records of gat_plan_timeline (an int64 array of shape [n, 8]) and
+        "ticks_per_ms"."""
         n = len(self.patterns)
         out = (C.c_uint64 * n)()
         st = CountStats()
@@ -285,10 +304,23 @@ class Plan:
         if graph_replays:
             check(lib().gat_count_graph(self._h, C.byref(cfg), int(graph_replays), out, C.byref(st)))
         else:
-            check(lib().gat_count(self._h, C.byref(cfg), _lib.FLAG_ACCOUNT if account else 0, out,
-                                  C.byref(st)))
-        stats = {k: getattr(st, k) for k, _ in CountStats._fields_ if k != "reserved"}
+            flags = (_lib.FLAG_ACCOUNT if account else 0) | (_lib.FLAG_TIMELINE if timeline else 0)
+            check(lib().gat_count(self._h, C.byref(cfg), flags, out, C.byref(st)))
+        stats = _stats_dict(st)
+        if timeline and not graph_replays:
+            stats["timeline"], stats["ticks_per_ms"] = self.timeline()
         return [int(x) for x in out], stats
+
+    def timeline(self):
+        """Records of the last count run with timeline=True: int64 [n, 8]
+        (launch, unit, start, end, build cycles, count cycles, items, pairs)
+        and wall-clock ticks per millisecond."""
+        import numpy as np
+        n, tpm = C.c_int64(), C.c_double()
+        check(lib().gat_plan_timeline(self._h, None, 0, C.byref(n), C.byref(tpm)))
+        buf = np.zeros((max(1, n.value), 8), dtype=np.int64)
+        check(lib().gat_plan_timeline(self._h, buf.ctypes.data, n.value, C.byref(n), C.byref(tpm)))
+        return buf[:n.value], tpm.value
 
     def count_info(self, i=0):
         """{"divisor", "multiplier", "relabeled", "restrictions": [(a, b), ...]}
@@ -301,7 +333,7 @@ class Plan:
         return {"divisor": int(d.divisor), "multiplier": int(d.multiplier), "relabeled": bool(d.relabeled),
                 "restrictions": [(d.restr_first[k], d.restr_second[k]) for k in range(d.n_restrictions)]}
 
-    def count_shard(self, root_begin, root_end, config=None, account=False):
+    def count_shard(self, root_begin, root_end, config=None, account=False, timeline=False):
         """Raw sums (one per pattern) of the embeddings whose Schedule vertex 0
         maps to a root in [root_begin, root_end) of the graph the plan runs on,
         and a stats dict. Over a partition of the roots the raw sums add up to
@@ -310,9 +342,12 @@ class Plan:
         out = (C.c_uint64 * n)()
         st = CountStats()
         cfg = config if isinstance(config, KernelConfig) else KernelConfig.from_dict(config or {})
-        check(lib().gat_count_shard(self._h, C.byref(cfg), _lib.FLAG_ACCOUNT if account else 0, int(root_begin),
-                                    int(root_end), out, C.byref(st)))
-        stats = {k: getattr(st, k) for k, _ in CountStats._fields_ if k != "reserved"}
+        flags = (_lib.FLAG_ACCOUNT if account else 0) | (_lib.FLAG_TIMELINE if timeline else 0)
+        check(lib().gat_count_shard(self._h, C.byref(cfg), flags, int(root_begin), int(root_end), out,
+                                    C.byref(st)))
+        stats = _stats_dict(st)
+        if timeline:
+            stats["timeline"], stats["ticks_per_ms"] = self.timeline()
         return [int(x) for x in out], stats
 
 

@@ -34,6 +34,7 @@ namespace gatk {
 
 constexpr int W = 64;
 constexpr int kSplitLevels = 3;  // LDS subtask levels (see KArgs)
+constexpr int kLevelsHbm = kSplitLevels + 1;  // timeline id of the HBM launch (its subtasks: + 1)
 
 // A split subtask: candidates [lo, hi) of the loop of node `node` (at depth
 // `depth` >= 2) below root edge `edge` (directed edge index: v0 = its row,
@@ -90,7 +91,28 @@ struct KArgs {
     long long in_base, out_base, sovf_base;
     long long in_cap, out_cap, sovf_cap;
     long long split_ticks;  // <= 0: no splitting
+    // Timeline (GAT_FLAG_TIMELINE): one record of kTlFields int64 per wave
+    // at tl + kTlFields * (tl_base + wave), or none when tl is null.
+    long long *tl;
+    int tl_base;
 };
+
+// Timeline record: launch id, unit (wave or workgroup), wall-clock start and
+// end, shader cycles spent building rows and counting (clique store), items
+// taken from the queue, and pairs counted (clique store).
+constexpr int kTlFields = 8;
+__device__ __forceinline__ void tl_write(long long *tl, int idx, int launch, int unit, long long t0, long long t1,
+                                         long long build, long long count, long long items, long long pairs) {
+    long long *r = tl + (long long)kTlFields * idx;
+    r[0] = launch;
+    r[1] = unit;
+    r[2] = t0;
+    r[3] = t1;
+    r[4] = build;
+    r[5] = count;
+    r[6] = items;
+    r[7] = pairs;
+}
 
 struct WaveState {
     unsigned long long task_acc[GATP_MAX_TERMS];
@@ -936,6 +958,8 @@ __global__ void __launch_bounds__(256) engine_kernel(const KArgs *__restrict__ a
         store = reinterpret_cast<int *>(smem + (size_t)wid * wave_bytes + sizeof(WaveState));
     Ctx C{a, P, S, store, ln};
     const int np = P.n_patterns;
+    const long long tl_t0 = wall_clock64();
+    long long tl_items = 0;
     for (int i = ln; i < np; i += W) S->wave_acc[i] = 0;
     if (ln == 0) {
         S->bytes = 0;
@@ -957,6 +981,7 @@ __global__ void __launch_bounds__(256) engine_kernel(const KArgs *__restrict__ a
             e0 = a.e_lo + uni64((long long)__shfl(r, 0));
             if (e0 >= a.e_hi) break;
         }
+        tl_items++;
         for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
         if (ln == 0) {
             S->abort = 0;
@@ -989,6 +1014,9 @@ __global__ void __launch_bounds__(256) engine_kernel(const KArgs *__restrict__ a
     for (int i = ln; i < np; i += W)
         if (S->wave_acc[i]) atomicAdd(&a.counts[i], S->wave_acc[i]);
     if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
+    if (a.tl && ln == 0)
+        tl_write(a.tl, a.tl_base + blockIdx.x * wpb + wid, GS ? kLevelsHbm : 0, blockIdx.x * wpb + wid, tl_t0,
+                 wall_clock64(), 0, 0, tl_items, 0);
 }
 
 // Subtask launches: list sub_mode (1 .. kSplitLevels: LDS buffers; then the
@@ -1009,6 +1037,8 @@ __global__ void __launch_bounds__(256) engine_sub_kernel(const KArgs *__restrict
         store = reinterpret_cast<int *>(smem + (size_t)wid * wave_bytes + sizeof(WaveState));
     Ctx C{a, P, S, store, ln};
     const int np = P.n_patterns;
+    const long long tl_t0 = wall_clock64();
+    long long tl_items = 0;
     for (int i = ln; i < np; i += W) S->wave_acc[i] = 0;
     if (ln == 0) {
         S->bytes = 0;
@@ -1023,6 +1053,7 @@ __global__ void __launch_bounds__(256) engine_sub_kernel(const KArgs *__restrict
         const unsigned long long total = made < (unsigned long long)a.in_cap ? made : (unsigned long long)a.in_cap;
         if (r >= total) break;
         const SplitTask *g = a.subs + a.in_base + (long long)r;
+        tl_items++;
         for (int i = ln; i < np; i += W) S->task_acc[i] = 0;
         if (ln == 0) {
             S->abort = 0;
@@ -1062,6 +1093,9 @@ __global__ void __launch_bounds__(256) engine_sub_kernel(const KArgs *__restrict
     for (int i = ln; i < np; i += W)
         if (S->wave_acc[i]) atomicAdd(&a.counts[i], S->wave_acc[i]);
     if (a.account && ln == 0 && S->bytes) atomicAdd(a.bytes, S->bytes);
+    if (a.tl && ln == 0)
+        tl_write(a.tl, a.tl_base + blockIdx.x * wpb + wid, a.sub_mode + (GS ? 1 : 0), blockIdx.x * wpb + wid, tl_t0,
+                 wall_clock64(), 0, 0, tl_items, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,6 +1156,8 @@ struct CArgs {
     int chunk;                   // vertices per main-queue grab
     int lin_thr;
     int gallop_ratio;
+    long long *tl;               // timeline records (GAT_FLAG_TIMELINE), or null
+    int tl_base;                 // first record of this launch
 };
 
 // Rows M_p of the list L (d entries) into rows (w words per row, already
@@ -1303,6 +1339,9 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     int *L = reinterpret_cast<int *>(base);
     unsigned *rows = reinterpret_cast<unsigned *>(L + a.cap + 4);  // L: d entries, then d + 1 offsets
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;  // per lane
+    const bool tl = a.tl != nullptr;
+    const long long tl_t0 = wall_clock64();
+    long long tl_build = 0, tl_count = 0, tl_items = 0, tl_pairs = 0;
     for (;;) {
         unsigned long long r = 0;
         if (ln == 0) r = atomicAdd(&a.queue[0], (unsigned long long)a.chunk);
@@ -1327,6 +1366,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                 }
                 continue;
             }
+            const long long c0 = tl ? clock64() : 0;
             const long long rb = a.off[v0];
             const int w = (d + 31) >> 5;
             for (int k = ln; k < d; k += W) L[k] = a.adj[rb + k];
@@ -1352,11 +1392,22 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                 acc[3] += (unsigned long long)run;
             }
             wsync();
+            const long long c1 = tl ? clock64() : 0;
             if (a.kmax >= 4) vtx_pairs(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
             wsync();
+            if (tl) {
+                tl_build += c1 - c0;
+                tl_count += clock64() - c1;
+                tl_items++;
+                tl_pairs += run;
+            }
         }
     }
     clique_flush(a, acc, bytes, ln);
+    if (tl && ln == 0) {
+        const int unit = blockIdx.x * (blockDim.x / W) + wid;
+        tl_write(a.tl, a.tl_base + unit, 0, unit, tl_t0, wall_clock64(), tl_build, tl_count, tl_items, tl_pairs);
+    }
 }
 
 // Workgroup tiers: the NW waves of a workgroup share one vertex of tier
@@ -1378,7 +1429,11 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     unsigned *rows_g = a.grows + (long long)blockIdx.x * a.hcap * hws;
     int *off_g = a.goff + (long long)blockIdx.x * (a.hcap + 1);
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;
+    const bool tl = a.tl != nullptr;
+    const long long tl_t0 = wall_clock64();
+    long long tl_build = 0, tl_count = 0, tl_items = 0, tl_pairs = 0, c0 = 0, c1 = 0;
     for (;;) {
+        if (tl) c0 = clock64();
         if (threadIdx.x == 0) {
             const unsigned long long r = atomicAdd(&a.queue[1 + TIER], 1ull);
             const unsigned long long cnt = a.ovf_n[TIER];
@@ -1436,13 +1491,26 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             if (part == 0) bytes += build_bytes;
             if (!LDS) __threadfence();
             __syncthreads();
+            if (tl) c1 = clock64();
             if (a.kmax >= 4)
                 vtx_pairs(a, (const unsigned *)rows, d, w, off, run, part * 64 * NW + (int)threadIdx.x,
                           64 * NW * parts, acc, bytes);
+            if (tl) {
+                tl_pairs += run;
+                tl_items++;
+            }
         }
         __syncthreads();
+        if (tl && c1) {
+            tl_build += c1 - c0;
+            tl_count += clock64() - c1;
+            c1 = 0;
+        }
     }
     clique_flush(a, acc, bytes, ln);
+    if (tl && threadIdx.x == 0)
+        tl_write(a.tl, a.tl_base + blockIdx.x, 1 + TIER, blockIdx.x, tl_t0, wall_clock64(), tl_build, tl_count,
+                 tl_items, tl_pairs);
 }
 
 __global__ void fill_u64(unsigned long long *p, long long n, unsigned long long v) {
@@ -1570,6 +1638,8 @@ struct PlanDevice {
     long long *ovf = nullptr;
     gatk::SplitTask *subs = nullptr;  // split subtasks (array store)
     gatk::KArgs *dargs = nullptr;     // arguments of the array-store launches of a count
+    long long *tl = nullptr;          // timeline records of the last GAT_FLAG_TIMELINE count
+    int tl_n = 0;                     // records that count reserved (0: none)
     long long *tier[3] = {};  // clique vertices of the workgroup tiers
     long long ovf_cap = 0;
     int *gstore = nullptr;
@@ -1592,7 +1662,7 @@ void destroy_plan_device(PlanDevice *d) {
     if (d->graph) (void)hipGraphDestroy(d->graph);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
-    for (void *p : {(void *)d->plan, (void *)d->dargs, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
+    for (void *p : {(void *)d->plan, (void *)d->dargs, (void *)d->tl, (void *)d->counts, (void *)d->queue, (void *)d->ovf_n, (void *)d->bytes,
                     (void *)d->err, (void *)d->ovf, (void *)d->subs, (void *)d->tier[0], (void *)d->tier[1], (void *)d->tier[2],
                     (void *)d->gstore, (void *)d->grows,
                     (void *)d->goff})
@@ -1679,6 +1749,8 @@ constexpr int kSplitUs = 2000;
 // device argument records of one array-store count: the main launch, the
 // LDS subtask levels, the HBM launch and the HBM subtask launch
 constexpr int kArgSlots = kLevels + 3;
+// timeline capacity (records of gatk::kTlFields int64 each)
+constexpr int kTlCap = 1 << 15;
 
 struct Launch {
     gatk::KArgs a;
@@ -1910,8 +1982,28 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
 }
 
 // Enqueue the whole count on the plan's stream (capturable: no memsets).
-static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
+static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events, bool timeline = false) {
     hipStream_t s = d->stream;
+    // timeline: one record per wave (array store, clique main launch) or per
+    // workgroup (clique tiers), launch after launch
+    long long *tl = nullptr;
+    int tl_next = 0;
+    auto tl_take = [&](int units) {  // first record of the next launch
+        const int b = tl_next;
+        tl_next += units;
+        return b;
+    };
+    if (timeline) {
+        const int waves = (int)(L.grid.x * (L.block.x / 64));
+        const long long need = L.clique ? (long long)waves + L.mid_grid.x + L.t1_grid.x + L.big_grid.x
+                                        : (long long)waves * (1 + (L.a.split_ticks > 0 ? kLevels : 0)) +
+                                              (long long)L.ovf_grid.x * (L.a.split_ticks > 0 ? 2 : 1);
+        if (need > kTlCap) GAT_FAIL(GAT_ELIMIT, "timeline needs %lld records (> %d)", need, kTlCap);
+        if (!d->tl) HIPCHECK(hipMalloc(&d->tl, sizeof(long long) * gatk::kTlFields * kTlCap));
+        hipLaunchKernelGGL(gatk::fill_u64, dim3(64), dim3(256), 0, s, (unsigned long long *)d->tl,
+                           (long long)gatk::kTlFields * kTlCap, ~0ull);
+        tl = d->tl;
+    }
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->counts, (long long)GATP_MAX_TERMS, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->queue, 16ll, 0ull);
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, d->ovf_n, 16ll, 0ull);
@@ -1919,29 +2011,37 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long *)d->err, 1ll, 0ull);
     gatk::KArgs a = L.a, ao = L.ao;
     a.account = ao.account = account ? 1 : 0;
+    a.tl = ao.tl = tl;
     ao.bytes = d->bytes + 1;  // second launch: its own W_alg counter
+    const int waves = (int)(L.grid.x * (L.block.x / 64));
     if (events) HIPCHECK(hipEventRecord(d->ev[0], s));
     gatk::CArgs c = L.ca;
     c.account = account ? 1 : 0;
+    c.tl = tl;
     if (L.clique) {
+        c.tl_base = tl_take(waves);
         hipLaunchKernelGGL(gatk::clique_kernel, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
     } else {
+        a.tl_base = tl_take(waves);
         hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, a, d->dargs);
         hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, d->dargs, L.wave_bytes);
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
     if (L.clique) {
         c.bytes = d->bytes + 1;  // workgroup tier 0: its own W_alg counter
+        c.tl_base = tl_take((int)L.mid_grid.x);
         hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW0>), L.mid_grid,
                            dim3(64 * gatk::CBLOCKW0), L.mid_lds, s, c);
         if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
         c.bytes = d->bytes + 2;  // tiers 1 and 2
+        c.tl_base = tl_take((int)L.t1_grid.x);
         if (L.t1_lds)
             hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW1>), L.t1_grid,
                                dim3(64 * gatk::CBLOCKW1), L.t1_lds, s, c);
         else
             hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW1>), L.t1_grid,
                                dim3(64 * gatk::CBLOCKW1), 0, s, c);
+        c.tl_base = tl_take((int)L.big_grid.x);
         hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1>), L.big_grid,
                            dim3(64 * gatk::CBLOCKW1), 0, s, c);
     } else {
@@ -1954,6 +2054,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
                 sk.out_n = k < kLevels ? d->ovf_n + 8 + k + 1 : nullptr;  // the last level does not split
                 sk.out_base = k * kSubCap;
                 sk.bytes = d->bytes + 1;
+                sk.tl_base = tl_take(waves);
                 hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, sk, d->dargs + k);
                 hipLaunchKernelGGL(gatk::engine_sub_kernel<false>, L.grid, L.block, L.lds, s, d->dargs + k,
                                    L.wave_bytes);
@@ -1961,6 +2062,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
         ao.out_n = d->ovf_n + 8 + kLevels + 1;  // the HBM launch splits into the HBM list
         ao.out_base = kLevels * kSubCap;
         ao.out_cap = 3 * kSubCap;
+        ao.tl_base = tl_take((int)L.ovf_grid.x);
         hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, ao, d->dargs + kLevels + 1);
         hipLaunchKernelGGL(gatk::engine_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s, d->dargs + kLevels + 1,
                            L.ovf_wave_bytes);
@@ -1971,6 +2073,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
             sh.in_base = kLevels * kSubCap;
             sh.in_cap = 3 * kSubCap;
             sh.out_n = nullptr;
+            sh.tl_base = tl_take((int)L.ovf_grid.x);
             hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, sh, d->dargs + kLevels + 2);
             hipLaunchKernelGGL(gatk::engine_sub_kernel<true>, L.ovf_grid, dim3(64), L.ovf_lds, s,
                                d->dargs + kLevels + 2, L.ovf_wave_bytes);
@@ -1979,6 +2082,7 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events) {
     }
     if (events) HIPCHECK(hipEventRecord(d->ev[2], s));
     HIPCHECK(hipGetLastError());
+    d->tl_n = timeline ? tl_next : 0;
     return GAT_OK;
 }
 
@@ -2025,6 +2129,10 @@ static int finish(gat_plan *p, const Launch &L, uint64_t *counts_out, gat_count_
         made += std::min<unsigned long long>(ovf[8 + kLevels + 1], (unsigned long long)(3 * kSubCap));
         st->split_tasks = L.clique ? 0 : (int32_t)(made - ovf[2]);
         st->split_overflow = L.clique ? 0 : (int32_t)ovf[2];
+        for (int k = 0; k < 4; k++)
+            st->split_level[k] = L.clique ? 0
+                                          : (int64_t)std::min<unsigned long long>(
+                                                ovf[8 + 1 + k], (unsigned long long)(k < kLevels ? kSubCap : 3 * kSubCap));
         st->alg_bytes = bytes[0] + bytes[1] + bytes[2];
         st->alg_bytes_main = bytes[0];
         st->alg_bytes_second = bytes[1];
@@ -2053,9 +2161,37 @@ extern "C" int gat_count(gat_plan *p, const gat_kernel_config *cfg, uint32_t fla
     Launch L;
     int rc = prepare(p, cfg, &L);
     if (rc) return rc;
-    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true);
+    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true, (flags & GAT_FLAG_TIMELINE) != 0);
     if (rc) return rc;
     return finish(p, L, counts_out, stats, true);
+}
+
+extern "C" int gat_plan_timeline(const gat_plan *p, int64_t *out, int64_t cap, int64_t *n_out,
+                                 double *ticks_per_ms) {
+    if (!p || !n_out) GAT_FAIL(GAT_EINVAL, "null plan or n_out");
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc) return rc;
+    PlanDevice *d = p->dev;
+    *n_out = 0;
+    if (ticks_per_ms) *ticks_per_ms = (double)di.wall_khz;
+    if (!d || !d->tl || d->tl_n == 0) return GAT_OK;
+    std::vector<long long> rec((size_t)d->tl_n * gatk::kTlFields);
+    HIPCHECK(hipStreamSynchronize(d->stream));
+    HIPCHECK(hipMemcpy(rec.data(), d->tl, rec.size() * sizeof(long long), hipMemcpyDeviceToHost));
+    long long t_min = LLONG_MAX;
+    for (int i = 0; i < d->tl_n; i++)
+        if (rec[(size_t)i * gatk::kTlFields] >= 0) t_min = std::min(t_min, rec[(size_t)i * gatk::kTlFields + 2]);
+    int64_t n = 0;
+    for (int i = 0; i < d->tl_n; i++) {
+        const long long *r = &rec[(size_t)i * gatk::kTlFields];
+        if (r[0] < 0) continue;  // a unit that wrote no record
+        if (out && n < cap)
+            for (int f = 0; f < gatk::kTlFields; f++) out[n * gatk::kTlFields + f] = (f == 2 || f == 3) ? r[f] - t_min : r[f];
+        n++;
+    }
+    *n_out = n;
+    return GAT_OK;
 }
 
 extern "C" int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags, int32_t root_begin,
@@ -2068,7 +2204,7 @@ extern "C" int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32
     int rc = prepare(p, cfg, &L);
     if (rc) return rc;
     set_roots(&L, *p->graph, root_begin, root_end);
-    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true);
+    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true, (flags & GAT_FLAG_TIMELINE) != 0);
     if (rc) return rc;
     return finish(p, L, raw_out, stats, true, true);
 }

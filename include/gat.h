@@ -164,9 +164,14 @@ typedef struct gat_count_stats {
     int32_t split_tasks;      /* array store: subtasks handed to the split
                                  launch                                      */
     int32_t split_overflow;   /* of those, re-run by the HBM launch          */
+    int64_t split_level[4];   /* array store: subtasks made for LDS levels 1,
+                                 2 and 3 and for the HBM list (which also
+                                 takes the split_overflow copies)             */
 } gat_count_stats;
 
 #define GAT_FLAG_ACCOUNT 1u   /* instrumented pass: accumulate W_alg bytes  */
+#define GAT_FLAG_TIMELINE 2u  /* record per-wave / per-workgroup start and end
+                                 times of every launch (gat_plan_timeline)  */
 
 /* Plan options (gat_plan_create_ex). Zero-initialised = gat_plan_create. */
 typedef struct gat_plan_options {
@@ -300,6 +305,18 @@ int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
                     int32_t root_begin, int32_t root_end, uint64_t *raw_out,
                     gat_count_stats *stats);
 void gat_plan_free(gat_plan *p);
+
+/* Records of the last gat_count run with GAT_FLAG_TIMELINE on this plan, 8
+ * int64 each: launch (clique store: 0 main, 1-3 workgroup tiers; array
+ * store: 0 main, 1-3 subtask levels, 4 HBM launch, 5 its subtasks), unit
+ * (wave of the launch, or workgroup of a clique tier), start and end in
+ * wall-clock ticks from the count's first start, shader cycles spent
+ * building bit rows and counting pairs (clique store), items taken from the
+ * launch's queue and pairs counted (clique store). Writes up to `cap`
+ * records, the number available to *n_out and the wall-clock ticks per
+ * millisecond to *ticks_per_ms. Load-imbalance evidence; no counterpart in
+ * the reference. */
+int gat_plan_timeline(const gat_plan *p, int64_t *out, int64_t cap, int64_t *n_out, double *ticks_per_ms);
 
 /* Text dump of a plan's compiled device program (debugging aid). Returns
  * the full length of the text (which may exceed cap) or an error code. */

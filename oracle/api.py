@@ -43,6 +43,8 @@ def lib():
         L.orc_clique_bitmap_walg_sample.restype = C.c_uint64
         L.orc_restricted_count.argtypes = [C.c_int32, P, P, C.c_int, P, C.c_int, P, P, C.c_int32, C.c_int32]
         L.orc_restricted_count.restype = C.c_uint64
+        L.orc_clique_counts_mt.argtypes = [C.c_int32, P, P, C.c_int, C.c_int32, C.c_int32, C.c_int, P]
+        L.orc_clique_counts_mt.restype = C.c_int
         L.orc_relabel.argtypes = [C.c_int32, P, P, P, P]
         L.orc_relabel.restype = None
         _lib = L
@@ -163,3 +165,15 @@ def clique_bitmap_walg_sample(off_relabeled, adj_relabeled, kmax, stride, first=
     c = lib().orc_clique_bitmap_walg_sample(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, kmax,
                                             stride, first, C.byref(w))
     return int(c), int(w.value)
+
+
+def clique_counts_mt(off, adj, kmax, threads, first=0, stride=1):
+    """[#K3, ..., #K_kmax] rooted at top vertices first, first + stride, ...
+    (the whole graph by default), counted on `threads` host threads."""
+    off, adj, n = _csr(off, adj)
+    out = (C.c_uint64 * 7)()
+    rc = lib().orc_clique_counts_mt(n, off.ctypes.data, adj.ctypes.data if len(adj) else None, kmax, first, stride,
+                                    threads, out)
+    if rc != 0:
+        raise ValueError("orc_clique_counts_mt failed (%d)" % rc)
+    return [int(out[k]) for k in range(3, kmax + 1)]

@@ -20,6 +20,9 @@
  *   orc_automorphisms       schedule.hpp:173-191
  *   orc_graph_cnt           schedule.hpp:86-170
  *   orc_clique_walg         algorithmic bytes of the oriented clique walk
+ *   orc_clique_counts_mt    whole-graph K3..K6 counts on all host cores
+ *                           (clique_mt.c), the checker and CPU baseline of
+ *                           the clique benchmark
  *
  * Pinning (what checks each function) is listed in oracle/README and in
  * tests/test_oracle.py.
@@ -67,6 +70,13 @@ uint64_t orc_clique_bitmap_walg(int32_t n, const int64_t *off, const int32_t *ad
 /* The same restricted to top vertices v0 = first, first + stride, ... */
 uint64_t orc_clique_bitmap_walg_sample(int32_t n, const int64_t *off, const int32_t *adj, int kmax, int32_t stride,
                                        int32_t first, uint64_t *walg);
+/* k-cliques K3..kmax (kmax <= 6) over the top vertices v0 = first,
+ * first + stride, ... (all of them with first 0, stride 1), on nthreads
+ * threads (clique_mt.c): counts7[k] = #K_k rooted there, for 3 <= k <= kmax.
+ * Any vertex labelling gives the same whole-graph counts. Returns 0, or < 0
+ * on bad arguments or allocation failure. */
+int orc_clique_counts_mt(int32_t n, const int64_t *off, const int32_t *adj, int kmax, int32_t first, int32_t stride,
+                         int nthreads, uint64_t *counts7);
 /* Relabel to descending degree (ties: ascending id). Output arrays must
  * hold n+1 offsets and off[n] entries. */
 void orc_relabel(int32_t n, const int64_t *off, const int32_t *adj, int64_t *off_out, int32_t *adj_out);

@@ -44,6 +44,26 @@ def test_rmat20_clique_properties(gat, gpu):
     assert both == [k4_bitmap[0], k5[0]]
 
 
+@pytest.mark.parametrize("scale", [12, 16, 20])
+def test_rmat_clique_counts_golden(gat, gpu, scale):
+    """Whole-graph K3, K4 and K5 of the benchmark graphs equal the oracle's
+    counts committed in tests/golden/clique_counts.json, batched and in the
+    tuned configuration of the benchmark."""
+    import json
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "clique_counts.json")) as f:
+        want = json.load(f)["rmat%d" % scale]
+    g = gat.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, 20)
+    assert (g.v_cnt(), g.undirected_edges()) == (want["vertices"], want["undirected_edges"])
+    with open(os.path.join(os.path.dirname(here), "graphautotune_amd", "tuning", "tuned_configs.json")) as f:
+        tuned = json.load(f)["rmat20-k4k5"]["config"]
+    plan = gat.Plan(g, [TRIANGLE, K4, K5], "unique")
+    for cfg in ({}, tuned):
+        got, _ = plan.count(cfg)
+        print("rmat%d" % scale, cfg, got, flush=True)
+        assert got == [want["K3"], want["K4"], want["K5"]], cfg
+
+
 def test_chung_lu_house_c5(gat, orc, gpu):
     from graphautotune_amd._lib import ScheduleData
     g = gat.Graph.chung_lu(600, 12.0, 2.1, 40, 21)

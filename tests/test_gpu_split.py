@@ -84,6 +84,13 @@ def test_split_path_is_taken(gat, gpu, graphs):
         got, st = plan.count(dict(SPLIT, **OVERFLOW))
         assert got == want and st["split_tasks"] > 0, p
         moved += st["split_overflow"]
+        lv = st["split_level"]
+        say("split levels", p, lv, st["split_overflow"])
+        # every level is reached: level-2 subtasks split into level 3, and
+        # the HBM launch (overflowed edges) hands subtasks to the HBM list
+        # beyond the copies of LDS-overflowed subtasks
+        assert lv[0] > 0 and lv[1] > 0 and lv[2] > 0, (p, lv)
+        assert lv[3] - st["split_overflow"] > 0, (p, lv, st["split_overflow"])
     assert moved > 0
 
 

@@ -88,7 +88,7 @@ def test_bin_rejects_asymmetric(gat, tmp_path):
     with pytest.raises(GatError, match="symmetric"):
         gat.Graph.load(p)
     # rows {1,2}, {0}, {0}: symmetric
-    adj2 =np.array([1, 2, 0, 0], dtype=np.int32)
+    adj2 = np.array([1, 2, 0, 0], dtype=np.int32)
     p.write_bytes(np.int32(3).tobytes() + np.int64(4).tobytes() + off.tobytes() + adj2.tobytes())
     assert gat.Graph.load(p).e_cnt() == 4
 
@@ -140,3 +140,19 @@ def test_triangle_count(gat, orc):
     dst = [j for i in range(m) for j in range(i + 1, m)]
     assert gat.Graph.from_edges(m, src, dst).triangles() == m * (m - 1) * (m - 2) // 6
     assert gat.Graph.from_edges(4, [0, 1, 2], [1, 2, 3]).triangles() == 0
+
+
+def test_golden_clique_graphs_unchanged(gat):
+    """The graphs behind tests/golden/clique_counts.json are still the ones
+    the generator makes (fingerprints of their CSR arrays)."""
+    import json
+    import os
+    import zlib
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "clique_counts.json")) as f:
+        gold = json.load(f)
+    for scale in (12, 16, 20):
+        g = gat.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, 20)  # csr() views g's memory: keep g alive
+        off, adj = g.csr()
+        want = gold["rmat%d" % scale]
+        assert zlib.adler32(np.ascontiguousarray(off).tobytes()) == want["adler32_offsets"]
+        assert zlib.adler32(np.ascontiguousarray(adj).tobytes()) == want["adler32_adj"]

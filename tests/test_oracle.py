@@ -358,3 +358,46 @@ def test_reference_count_roots_partitions_the_count(orc):
         cuts = [0, 1, 7, 20, 39, 40]
         parts = [orc.reference_count_roots(off, adj, s, lo, hi) for lo, hi in zip(cuts, cuts[1:])]
         assert all(o for _, o in parts) and sum(v for v, _ in parts) == full, p
+
+
+def test_clique_counts_mt_against_brute_force_and_closed_forms(orc):
+    """The whole-graph clique counter (all host cores): brute-force unique
+    counts on random graphs in input and degree order (the count does not
+    depend on the labelling), C(m, k) on complete graphs (rows wider than
+    one 64-bit word at m = 150), nothing on a bipartite graph, and stride
+    samples that partition the roots; any thread count gives one result."""
+    for seed, (n, p) in enumerate([(40, 0.3), (70, 0.2), (90, 0.12)]):
+        off, adj = random_graph(n, p, 20 + seed)
+        want = [orc.unique_count(off, adj, clique(k)) for k in (3, 4, 5, 6)]
+        assert orc.clique_counts_mt(off, adj, 6, 3) == want
+        ro, ra = orc.relabel(off, adj)
+        assert orc.clique_counts_mt(ro, ra, 6, 1) == want
+        assert orc.clique_counts_mt(ro, ra, 4, 8) == want[:2]
+        parts = [orc.clique_counts_mt(ro, ra, 6, 2, first=f, stride=5) for f in range(5)]
+        assert [sum(c) for c in zip(*parts)] == want
+    for m in (7, 66, 150):
+        off, adj = complete(m)
+        assert orc.clique_counts_mt(off, adj, 6, 4) == [math.comb(m, k) for k in (3, 4, 5, 6)]
+    a, b = 9, 13
+    off, adj = csr_of_edges(a + b, [(i, a + j) for i in range(a) for j in range(b)])
+    assert orc.clique_counts_mt(off, adj, 6, 2) == [0, 0, 0, 0]
+    off, adj = csr_of_edges(5, [])
+    assert orc.clique_counts_mt(off, adj, 5, 2) == [0, 0, 0]
+
+
+def test_golden_clique_counts_small_scales(gat, orc):
+    """The committed clique counts of R-MAT scales 12 and 16 are what the
+    oracle computes now (scale 20 is checked against the GPU and by the
+    benchmark's CPU baseline)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "clique_counts.json")) as f:
+        gold = json.load(f)
+    for scale in (12, 16):
+        g = gat.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, 20)  # csr() views g's memory: keep g alive
+        off, adj = g.csr()
+        want = gold["rmat%d" % scale]
+        assert orc.clique_counts_mt(off, adj, 5, 4) == [want["K3"], want["K4"], want["K5"]]
+    g = gat.Graph.rmat(12, 16, 0.57, 0.19, 0.19, 20)
+    off, adj = g.csr()
+    assert orc.unique_count(off, adj, K4) == gold["rmat12"]["K4"]
