@@ -317,6 +317,114 @@ __device__ __forceinline__ int wave_isect(SP stream, int slen, PR probe, OP out,
     return total;
 }
 
+// Merge-path intersection of two sorted lists A[0,na) and B[0,nb), for
+// pairs of similar length. In merge order A[i] precedes B[j] iff
+// A[i] <= B[j]; the 64 lanes split the na + nb merged positions into equal
+// runs, find where their run starts by a binary search along its diagonal,
+// and merge their run serially. A common value x = A[i] = B[j] is found
+// once, when B[j] is consumed and the A element before it (A[i-1], possibly
+// consumed by the previous lane) equals it. Sorted output (MAT): one pass
+// counts, an exclusive scan of the lanes' counts places each run's hits,
+// and a second pass writes them.
+template <bool MAT, class PA, class PB, class OP>
+__device__ __forceinline__ int wave_merge_isect(PA A, int na, PB B, int nb, OP out, const int *ex, int nex, int ln) {
+    const int total = na + nb;
+    if (na == 0 || nb == 0) return 0;
+    const int run = (total + W - 1) / W;
+    const int k0 = min(total, ln * run), k1 = min(total, k0 + run);
+    int lo = max(0, k0 - nb), hi = min(k0, na);
+    while (lo < hi) {  // i = number of A elements among the first k0 merged
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] <= B[k0 - 1 - mid]) lo = mid + 1; else hi = mid;
+    }
+    const int i0 = lo, j0 = k0 - lo;
+    int cnt = 0;
+    for (int pass = 0; pass < (MAT ? 2 : 1); pass++) {
+        int i = i0, j = j0, w = 0;
+        if (pass == 1) {
+            int tot = 0;
+            w = excl_scan(cnt, &tot);
+        }
+        int prev = i > 0 ? A[i - 1] : INT_MIN;  // the A element merged last
+        int a = i < na ? A[i] : INT_MAX;
+        for (int k = k0; k < k1; k++) {
+            if (i < na && (j >= nb || a <= B[j])) {
+                prev = a;
+                i++;
+                a = i < na ? A[i] : INT_MAX;
+            } else {
+                const int x = B[j++];
+                bool hit = prev == x;
+                for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
+                if (hit) {
+                    if (pass == 0) cnt++;
+                    else out[w++] = x;
+                }
+            }
+        }
+    }
+    int tot = 0;
+    (void)excl_scan(cnt, &tot);
+    return tot;
+}
+
+// Galloping intersection of a short sorted list S[0,ns) with a long one
+// L[0,nl), for skewed pairs. Each lane takes a block of up to 64
+// consecutive elements of S (rounds of 64 blocks), finds the first by a
+// binary search of L and every further one by an exponential search from
+// the previous position (steps 1, 2, 4, ... then a binary search of the
+// last step), so an element costs about 2 log2 of its gap in L instead of
+// log2(nl). Hits are kept as a per-lane bit mask; the exclusive scan of
+// their counts places the sorted output (MAT).
+template <bool MAT, class PS, class PL, class OP>
+__device__ __forceinline__ int wave_gallop_isect(PS S, int ns, PL L, int nl, OP out, const int *ex, int nex, int ln) {
+    int total = 0;
+    for (int base = 0; base < ns; base += W * W) {
+        const int blk = min(W, (ns - base + W - 1) / W);
+        const int b0 = min(ns, base + ln * blk), b1 = min(ns, b0 + blk);
+        unsigned long long mask = 0;
+        int pos = 0;
+        for (int e = b0; e < b1; e++) {
+            const int x = S[e];
+            int lo, hi;  // the first element >= x lies in [lo, hi]
+            if (e == b0) {
+                lo = 0;
+                hi = nl;
+            } else {
+                lo = pos;
+                int step = 1;
+                hi = pos;
+                while (hi < nl && L[hi] < x) {
+                    lo = hi + 1;
+                    hi = pos + step;
+                    step <<= 1;
+                }
+                hi = min(hi, nl);
+            }
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L[mid] < x) lo = mid + 1; else hi = mid;
+            }
+            pos = lo;
+            bool hit = pos < nl && L[pos] == x;
+            for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
+            if (hit) mask |= 1ull << (e - b0);
+        }
+        int tot = 0;
+        const int w = excl_scan(__popcll(mask), &tot);
+        if (MAT) {
+            int o = total + w;
+            while (mask) {
+                const int b = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                out[o++] = S[b0 + b];
+            }
+        }
+        total += tot;
+    }
+    return total;
+}
+
 // Stream A (the new vertex's neighbour list) and probe the father set, or
 // the reverse? A side is cheap to probe when it sits in LDS or has a hub
 // bitmap; the other side is then streamed unless it is gallop_ratio times
@@ -407,7 +515,30 @@ __device__ __forceinline__ void exec_op_uniform(const Ctx &C, const GatOp &o, in
         return o.kind == GATP_LIST ? wave_isect<true>(stream, slen, probe, out, ex, nex, C.ln)
                                    : wave_isect<false>(stream, slen, probe, out, ex, nex, C.ln);
     };
-    if (!has_r && o.kind == GATP_COUNT) {
+    // merge path or galloping between two sorted arrays (strategies 3 and 4;
+    // a hub's bitmap is not used there)
+    const int sorted_pair = has_r && (a.strategy == 3 || a.strategy == 4) ? a.strategy : 0;
+    auto run2 = [&](auto P, int np_, auto Q, int nq) {  // P, Q: sorted arrays
+        if (sorted_pair == 3)
+            return o.kind == GATP_LIST ? wave_merge_isect<true>(P, np_, Q, nq, out, ex, nex, C.ln)
+                                       : wave_merge_isect<false>(P, np_, Q, nq, out, ex, nex, C.ln);
+        // galloping: the shorter list is walked, the longer one searched
+        if (np_ <= nq)
+            return o.kind == GATP_LIST ? wave_gallop_isect<true>(P, np_, Q, nq, out, ex, nex, C.ln)
+                                       : wave_gallop_isect<false>(P, np_, Q, nq, out, ex, nex, C.ln);
+        return o.kind == GATP_LIST ? wave_gallop_isect<true>(Q, nq, P, np_, out, ex, nex, C.ln)
+                                   : wave_gallop_isect<false>(Q, nq, P, np_, out, ex, nex, C.ln);
+    };
+    if (sorted_pair) {
+        if (b_lds) {
+            const int *B = C.store + (long long)C.P.slots[f].store * a.cap;
+            result = run2(A, alen, B, blen);
+        } else {
+            const int *B = C.P.slots[f].kind == GATP_ALIAS ? a.adj + bbeg
+                                                           : C.store + (long long)C.P.slots[f].store * a.cap;
+            result = run2(A, alen, B, blen);
+        }
+    } else if (!has_r && o.kind == GATP_COUNT) {
         result = alen - uni(excluded_in_list(a, c, ab, alen, ex, nex));
     } else if (!has_r) {
         result = run(A, alen, [](int) { return true; });
@@ -1789,7 +1920,7 @@ static void fill_defaults(gat_kernel_config *c) {
     if (c->waves_per_cu <= 0) c->waves_per_cu = 8;
     if (c->queue_chunk <= 0) c->queue_chunk = 16;
     if (c->slot_capacity <= 0) c->slot_capacity = 1024;
-    if (c->intersect_strategy < 0 || c->intersect_strategy > 2) c->intersect_strategy = 0;
+    if (c->intersect_strategy < 0 || c->intersect_strategy > 4) c->intersect_strategy = 0;
     if (c->gallop_ratio <= 0) c->gallop_ratio = 32;
     if (c->search_linear_threshold < 0) c->search_linear_threshold = 0;
     if (c->hub_degree == 0) c->hub_degree = 256;

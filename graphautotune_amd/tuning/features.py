@@ -7,7 +7,7 @@ import numpy as np
 
 from .space import SPACE
 
-FEATURES = 96
+FEATURES = 128
 MAXN = 7
 
 

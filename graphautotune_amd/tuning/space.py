@@ -24,8 +24,8 @@ SPACE = [
     Param("queue_chunk", [1, 4, 16, 64], 16, "edges per work-queue grab (prefix chunk)"),
     Param("slot_capacity", [32, 64, 128, 256, 512, 1024], 1024,
           "LDS set-buffer entries (array store); per-wave row limit d <= min(256, value) (bitmap store)"),
-    Param("intersect_strategy", [0, 1, 2], 0, "auto / stream-and-search / stream-shorter"),
-    Param("gallop_ratio", [4, 32, 256], 32, "skew ratio that switches to galloping"),
+    Param("intersect_strategy", [0, 1, 2, 3, 4], 0, "auto / stream-and-search / stream-shorter / merge path / galloping"),
+    Param("gallop_ratio", [4, 32, 256], 32, "auto: skew ratio above which the slow side is not streamed"),
     Param("search_linear_threshold", [0, 4, 16, 64, INT_MAX], 0, "SetSearchType (details.py:11-13)"),
     Param("vertex_store", [0, 1], 0, "VertexStoreType (details.py:21-23): auto(bitmap for cliques) / array"),
 ]

@@ -123,8 +123,25 @@ typedef struct gat_kernel_config {
     int32_t slot_capacity;    /* int32 entries per LDS prefix-set slot         */
     int32_t leaf_mode;        /* 0 flattened, else lanes per candidate 1..64
                                  (threads_per_warp, engine_config.hpp:6)      */
-    int32_t intersect_strategy; /* 0 auto, 1 stream-and-search, 2 gallop      */
-    int32_t gallop_ratio;     /* auto: gallop when |long| > ratio * |short|   */
+    int32_t intersect_strategy; /* how one wave intersects two sorted lists:
+                                 0 auto: probe the side that sits in LDS or
+                                   has a hub bitmap, else stream the shorter
+                                   list (binary search in the other);
+                                 1 stream the new vertex's list, search the
+                                   other;
+                                 2 stream-shorter: stream the shorter list,
+                                   binary search in the longer;
+                                 3 merge path: the lanes split the merged
+                                   order of both lists into equal runs;
+                                 4 galloping: lanes walk blocks of the shorter
+                                   list with exponential searches from their
+                                   previous position in the longer.
+                                 3 and 4 apply to the wave-wide intersections
+                                 of non-leaf levels; leaf levels (a flattened
+                                 sweep over many candidates) use 0 there   */
+    int32_t gallop_ratio;     /* auto: stream the side that is not in LDS or a
+                                 hub only when it is at most this many times
+                                 longer than that side                       */
     int32_t search_linear_threshold; /* set_search_type: linear scan below this
                                  size, binary above (array_vertex_set.cuh:145) */
     int32_t max_regs;         /* register cap hint: 0, 64 or 128 VGPRs         */

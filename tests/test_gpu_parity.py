@@ -20,6 +20,8 @@ CONFIGS = [
     {"slot_capacity": 8, "queue_chunk": 1},  # tiny LDS buffers: most tasks take the HBM overflow launch
     {"queue_chunk": 64, "intersect_strategy": 1, "block_waves": 2, "waves_per_cu": 4},
     {"intersect_strategy": 2, "search_linear_threshold": 16, "gallop_ratio": 2},
+    {"intersect_strategy": 3, "vertex_store": 1},  # merge path (cliques on the array store too)
+    {"intersect_strategy": 4, "vertex_store": 1, "slot_capacity": 8, "queue_chunk": 1},  # galloping, HBM buffers
 ]
 
 
@@ -228,3 +230,21 @@ def test_walg_and_counts_exact_under_overflow(gat, gpu):
         assert c1 == c2, pats
         assert (s2["overflow_tasks"] > 0) == overflows, pats
         assert s1["alg_bytes"] == s2["alg_bytes"], (pats, s1["alg_bytes"], s2["alg_bytes"])
+
+
+@pytest.mark.parametrize("strategy", [0, 1, 2, 3, 4])
+def test_intersection_strategies_long_lists(gat, orc, gpu, strategy):
+    """Hub-heavy Chung-Lu graph (degrees up to ~2,000): the array store's
+    wave-wide intersections of long lists, materialised (K4, K5: the depth-1
+    list) and counted (triangles), with every strategy, against the oracle's
+    clique counts; blocks of many elements per lane (galloping) and long
+    merge runs are exercised. Hub bitmaps off so every probe is a search."""
+    g = gat.Graph.chung_lu(20000, 30.0, 2.1, 2000, 13)
+    off, adj = g.csr()
+    want = orc.clique_counts_mt(off, adj, 5, 8)
+    plan = gat.Plan(g, [TRIANGLE, K4, K5], "unique")
+    for extra in ({}, {"slot_capacity": 64}):
+        cfg = dict({"vertex_store": 1, "intersect_strategy": strategy, "hub_degree": -1}, **extra)
+        got, _ = plan.count(cfg)
+        say("strategy", strategy, extra, got, want)
+        assert got == want, (cfg, got, want)
