@@ -10,16 +10,16 @@ reports progress after each shard. The same loop, over a dealt-out subset
 of the ranges, is one GPU's share when a count is split across GPUs.
 
     python -m graphautotune_amd.sharded --graph cl4194304 --pattern house
+    python -m graphautotune_amd.sharded --graph rmat22 --batched-set --estimate
 """
 import argparse
 import json
 import sys
 import time
 
-from . import core
+from . import core, workloads
 
-PATTERNS = {"house": "0100110110010110110010100", "c5": "0100110100010100010110010",
-            "k4": "0111101111011110", "k5": "0111110111110111110111110"}
+PATTERNS = workloads.NAMED
 
 
 def raw_sharded(plan, config=None, target_s=5.0, first=1, roots=None, log=None):
@@ -61,25 +61,94 @@ def count_sharded(plan, config=None, target_s=5.0, first=1, log=None):
     return finish_counts(plan, raw), shards
 
 
+def strata_of(n, first=1):
+    """Root strata [lo, hi) of a degree-ordered graph: [0, first), then
+    ranges that double in size, so each holds roots of similar degree."""
+    cuts, lo, size = [], 0, max(1, first)
+    while lo < n:
+        hi = min(n, lo + size)
+        cuts.append((lo, hi))
+        lo, size = hi, size * 2
+    return cuts
+
+
+def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=0, log=None):
+    """Estimated whole-graph raw sums and device time of `plan` from a
+    stratified sample of root blocks, with standard errors.
+
+    Roots are ordered by degree (unique and labeled plans run on the
+    degree-ordered graph), so the strata of strata_of() hold roots of
+    similar cost. A stratum is cut into at most max_units blocks of
+    consecutive roots; per_stratum of them (seeded, without replacement)
+    are counted with gat_count_shard, or all of them when there are no
+    more. Per stratum h with N_h blocks, n_h sampled, block sums y: total
+    N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) / n_h (exact when
+    n_h = N_h). Returns {"raw": [...], "raw_se": [...], "seconds",
+    "seconds_se", "strata": [...]} (raw sums are not divided by the
+    divisor)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = plan.graph.v_cnt()
+    np_ = len(plan.patterns)
+    raw, var = np.zeros(np_), np.zeros(np_)
+    sec, sec_var = 0.0, 0.0
+    strata = []
+    for lo, hi in strata_of(n):
+        size = hi - lo
+        blk = -(-size // max_units)  # ceil: roots per block
+        nb = -(-size // blk)
+        pick = np.arange(nb) if nb <= per_stratum else np.sort(rng.choice(nb, per_stratum, replace=False))
+        ys, ts = [], []
+        for b in pick:
+            r0 = lo + int(b) * blk
+            got, st = plan.count_shard(r0, min(hi, r0 + blk), config)
+            ys.append(got)
+            ts.append(st["seconds"])
+        ys, ts = np.array(ys, dtype=np.float64), np.array(ts)
+        k = len(pick)
+        tot = nb * ys.mean(axis=0)
+        fpc = 1.0 - k / nb
+        v = (nb * nb * fpc * ys.var(axis=0, ddof=1) / k) if k > 1 and fpc > 0 else np.zeros(np_)
+        tsum = nb * ts.mean()
+        tv = nb * nb * fpc * ts.var(ddof=1) / k if k > 1 and fpc > 0 else 0.0
+        raw += tot
+        var += v
+        sec += tsum
+        sec_var += tv
+        strata.append({"roots": [lo, hi], "block": blk, "blocks": nb, "sampled": k,
+                       "raw": tot.tolist(), "raw_se": np.sqrt(v).tolist(), "seconds": tsum,
+                       "seconds_se": float(np.sqrt(tv)), "sampled_seconds": float(ts.sum())})
+        if log:
+            log("stratum [%d, %d): %d of %d blocks of %d roots, %.3f s sampled, projected %.2f +- %.2f s"
+                % (lo, hi, k, nb, blk, ts.sum(), tsum, np.sqrt(tv)))
+    return {"raw": raw.tolist(), "raw_se": np.sqrt(var).tolist(), "seconds": sec,
+            "seconds_se": float(np.sqrt(sec_var)), "strata": strata}
+
+
 def make_graph(spec):
-    """rmatS: R-MAT scale S, edge factor 16, (.57,.19,.19,.05), seed 20;
-    clN: Chung-Lu, N vertices, mean degree 24, exponent 2.1, cap 50,000,
-    seed 21; or a path to a .bin graph."""
-    if spec.startswith("rmat") and spec[4:].isdigit():
-        return core.GlobalMemoryGraph.rmat(int(spec[4:]), 16, 0.57, 0.19, 0.19, 20)
-    if spec.startswith("cl") and spec[2:].isdigit():
-        return core.GlobalMemoryGraph.chung_lu(int(spec[2:]), 24.0, 2.1, 50000, 21)
-    return core.GlobalMemoryGraph.load(spec)
+    """A workload graph by name (workloads.graph: erN, rmatS with seed S,
+    clN) or a path to a .bin graph."""
+    try:
+        return workloads.graph(spec)
+    except ValueError:
+        return core.GlobalMemoryGraph.load(spec)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--graph", required=True, help="rmatS, clN or a .bin file")
-    ap.add_argument("--pattern", action="append", required=True,
+    ap.add_argument("--pattern", action="append", default=[],
                     help="adjacency string or one of %s (repeatable)" % ", ".join(sorted(PATTERNS)))
+    ap.add_argument("--batched-set", action="store_true",
+                    help="add the batched configuration's 31 patterns (workloads.batched_patterns)")
     ap.add_argument("--mode", default="unique", choices=["unique", "labeled", "reference"])
     ap.add_argument("--config", default=None, help="JSON kernel config")
     ap.add_argument("--target", type=float, default=5.0, help="seconds per shard")
+    ap.add_argument("--estimate", action="store_true",
+                    help="project the whole-graph counts and device time from a stratified sample of root "
+                         "blocks (see stratified_estimate) instead of counting every root")
+    ap.add_argument("--per-stratum", type=int, default=24, help="with --estimate: blocks sampled per stratum")
+    ap.add_argument("--seed", type=int, default=0, help="with --estimate: sampling seed")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="count only roots [LO, HI) and report raw sums (add the raw sums of a partition, "
                          "then divide by the divisor and multiply by the multiplier)")
@@ -89,9 +158,25 @@ def main(argv=None):
     g = make_graph(args.graph)
     log("graph %s: %d vertices, %d edges (%.1f s)" % (args.graph, g.v_cnt(), g.undirected_edges(),
                                                    time.perf_counter() - t0))
-    pats = [PATTERNS.get(p, p) for p in args.pattern]
+    pats = [PATTERNS.get(p, p) for p in args.pattern] + (workloads.batched_patterns() if args.batched_set else [])
+    if not pats:
+        ap.error("no patterns: give --pattern or --batched-set")
     plan = core.Plan(g, pats, args.mode)
     cfg = json.loads(args.config) if args.config else {}
+    if args.estimate:
+        est = stratified_estimate(plan, cfg, per_stratum=args.per_stratum, seed=args.seed, log=log)
+        infos = [plan.count_info(i) for i in range(len(pats))]
+        E = g.undirected_edges()
+        print(json.dumps({
+            "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
+            "kernel_config": cfg, "per_stratum": args.per_stratum, "seed": args.seed,
+            "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
+            "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],
+            "projected_device_seconds": est["seconds"], "projected_device_seconds_se": est["seconds_se"],
+            "projected_edges_per_s": E / est["seconds"] if est["seconds"] > 0 else None,
+            "sampled_device_seconds": sum(st["sampled_seconds"] for st in est["strata"]),
+            "strata": est["strata"]}), flush=True)
+        return
     roots = tuple(int(x) for x in args.roots.split(":")) if args.roots else None
     raw, shards = raw_sharded(plan, cfg, args.target, 1, roots, log)
     dev = sum(s for _, _, s in shards)

@@ -6,33 +6,25 @@ reference's Schedules, at identical counts.
 
     python -m graphautotune_amd.tuning.batched_study --proxy-scale 10 --scale 12
 
-The 6-vertex patterns are drawn as the tests draw them: numpy
-default_rng(6), 10 of the 112 connected 6-vertex graphs listed in
-tests/golden/connected_patterns.txt.
+The pattern set is workloads.batched_patterns(): the 21 connected 5-vertex
+patterns and 10 of the connected 6-vertex ones of edge density 0.4-0.8,
+drawn by numpy's default_rng(6).
 """
 import argparse
 import json
-import os
 import sys
 import time
 
-import numpy as np
-
+from .. import workloads
 from ..core import Graph, Plan
 from .schedules import tune_schedule_ranks
 
-ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-
-
 def connected_patterns(n):
-    with open(os.path.join(ROOT, "tests", "golden", "connected_patterns.txt")) as f:
-        return [line.split()[1] for line in f if line.strip() and int(line.split()[0]) == n]
+    return list(workloads.connected_patterns(n))
 
 
 def pattern_set():
-    rng = np.random.default_rng(6)
-    p6 = connected_patterns(6)
-    return connected_patterns(5) + [p6[i] for i in rng.choice(len(p6), 10, replace=False)]
+    return workloads.batched_patterns()
 
 
 def timed(plan, config=None):

@@ -222,13 +222,12 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
-    from .batched_study import connected_patterns
-    p6 = connected_patterns(6)
-    rng = np.random.default_rng(6)
-    held = [p6[i] for i in rng.choice(len(p6), 10, replace=False)]  # the batched set's 6-vertex patterns
+    from .. import workloads
+    p6 = list(workloads.connected_patterns(6))
+    held = workloads.batched_patterns()[21:]  # the batched set's 6-vertex patterns
     train6 = [p for p in p6 if p not in held]
-    train = connected_patterns(5) + [train6[i] for i in np.random.default_rng(7).choice(len(train6), 24,
-                                                                                          replace=False)]
+    train = list(workloads.connected_patterns(5)) + [
+        train6[i] for i in np.random.default_rng(7).choice(len(train6), 24, replace=False)]
     t0 = time.time()
     X, y, meta = collect_schedules([int(s) for s in args.train_scales.split(",")], train, args.max_rank, log=log)
     log("collected %d timings in %.1f s" % (len(y), time.time() - t0))

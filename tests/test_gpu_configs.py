@@ -79,10 +79,8 @@ def test_chung_lu_house_c5(gat, orc, gpu):
 
 
 def test_batched_21_plus_10(gat, orc, gpu):
-    rng = np.random.default_rng(6)
-    p6 = connected_patterns(6)
-    six = [p6[i] for i in rng.choice(len(p6), 10, replace=False)]
-    pats = connected_patterns(5) + six
+    from graphautotune_amd import workloads
+    pats = workloads.batched_patterns()
     g = gat.Graph.rmat(9, 16, 0.57, 0.19, 0.19, 22)
     batched, _ = gat.Plan(g, pats, "unique").count()
     plain = [gat.Plan(g, [p], "unique").count()[0][0] for p in pats]

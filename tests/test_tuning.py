@@ -17,7 +17,7 @@ def test_space_matches_reference_values():
     # tuning/tuning/config/details.py:11-13 (SetSearchType) kept verbatim
     assert sp.BY_NAME["search_linear_threshold"].values == [0, 4, 16, 64, 2 ** 31 - 1]
     full = sp.ConfigSpace()
-    assert full.size() == 3 * 4 * 4 * 6 * 3 * 3 * 5 * 2
+    assert full.size() == 3 * 4 * 4 * 6 * 5 * 3 * 5 * 2  # intersect_strategy: 5 values
     assert len(list(sp.ConfigSpace(["block_waves", "queue_chunk"]).all())) == 12
 
 
@@ -109,3 +109,19 @@ def test_manipulator_drops_infeasible_runs():
     bad = sp.Config({"block_waves": 4, "slot_capacity": 1024})
     xs, ys = m.run_batch([bad, sp.Config()])
     assert xs == [sp.Config()] and ys[0] < FLOAT_INF
+
+
+def test_workload_patterns():
+    """The product's pattern enumeration equals the committed fixture (and
+    OEIS A001349: 2, 6, 21, 112 connected graphs on 3..6 vertices); the
+    batched set is the 21 five-vertex patterns plus 10 distinct six-vertex
+    patterns of edge density 0.4-0.8."""
+    from graphautotune_amd import workloads
+    from patterns import connected_patterns
+    for n, count in ((3, 2), (4, 6), (5, 21), (6, 112)):
+        assert list(workloads.connected_patterns(n)) == connected_patterns(n)
+        assert len(connected_patterns(n)) == count
+    b = workloads.batched_patterns()
+    assert len(b) == 31 == len(set(b)) and b[:21] == connected_patterns(5)
+    assert all(len(p) == 36 and 0.4 <= workloads.edge_density(p) <= 0.8 for p in b[21:])
+    assert workloads.batched_patterns() == b  # seeded
