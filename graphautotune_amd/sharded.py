@@ -104,6 +104,8 @@ def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=
             got, st = plan.count_shard(r0, min(hi, r0 + blk), config)
             ys.append(got)
             ts.append(st["seconds"])
+            if log and st["seconds"] > 1.0:
+                log("  block [%d, %d): %.2f s" % (r0, min(hi, r0 + blk), st["seconds"]))
         ys, ts = np.array(ys, dtype=np.float64), np.array(ts)
         k = len(pick)
         tot = nb * ys.mean(axis=0)

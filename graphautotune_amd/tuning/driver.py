@@ -21,22 +21,31 @@ class EngineBug(RuntimeError):
 
 
 class Driver:
-    def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1):
+    """roots=(lo, hi) times a root shard (gat_count_shard, raw sums) instead
+    of the whole graph: on large graphs a fixed shard is the workload."""
+
+    def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1, roots=None):
         self.plan = core.Plan(graph, patterns, mode)
         self.graph = graph
         self.patterns = list(patterns)
         self.reps, self.warmup = reps, warmup
+        self.roots = roots
         self.counts = None
         self.log = []
+
+    def _count(self, cfg):
+        if self.roots:
+            return self.plan.count_shard(self.roots[0], self.roots[1], cfg)
+        return self.plan.count(cfg)
 
     def run(self, config):
         cfg = config.as_dict() if hasattr(config, "as_dict") else dict(config)
         try:
             for _ in range(self.warmup):
-                self.plan.count(cfg)
+                self._count(cfg)
             secs = []
             for _ in range(self.reps):
-                c, st = self.plan.count(cfg)
+                c, st = self._count(cfg)
                 if self.counts is None:
                     self.counts = c
                 elif c != self.counts:
