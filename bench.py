@@ -6,11 +6,16 @@ and the launches after it (the workgroup tiers of the bitmap clique store).
 `value` is undirected edges / median device time of a batched step (inputs
 resident in HBM). The plain single-pattern engine (default config) is timed
 beside it; `batched_vs_plain` = (t_K4 + t_K5) / t_batched at identical
-counts. `roofline` is for whichever of the main and the second launch takes
-longer: W_alg bytes of that launch (instrumented pass) / its hipEvent-timed
-duration, against 8 TB/s; `hbm_achieved` is the same launch's measured HBM
-bytes (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel) over
-that duration. `cpu_baseline` times the CPU oracle's whole-graph K4 + K5
+counts. `roofline` is for the launch that takes longest (hipEvents inside
+the timed steps). The bitmap clique store works on bit rows in LDS and is
+charged against the vector ALU (rocprofv3 counters in
+profiles/clique_rmat20_k4k5.json give its issue rate, lane use and LDS use):
+`achieved` = W_alg / 4 algorithmic integer operations of that launch
+(instrumented pass) / its duration, against the VALU's 78.6 T lane-ops/s;
+`traffic` is the same launch's HBM bytes (rocprofv3 FETCH_SIZE +
+WRITE_SIZE, calibrated on a copy kernel) and `hbm_frac` their rate against
+8 TB/s. The array store's roofline is W_alg bytes against 8 TB/s.
+`cpu_baseline` times the CPU oracle's whole-graph K4 + K5
 count on the host's cores (at most 16) and checks its counts against the
 GPU's; `naive_c1` times the reference's own CPU path (naive_pm, restated) on
 its triangle workload, one thread.
@@ -31,6 +36,64 @@ METRIC = "exact-count throughput (edges/s), batched+tuned vs plain, R-MAT s20 4/
 K4 = "0111101111011110"
 K5 = "0111110111110111110111110"
 HBM_PEAK_GBPS = 8000.0
+# MI355X vector ALU: 157.3 TFLOP/s FP32 vector peak (AMD) / 2 flops per FMA
+# = 78.6 T lane-operations/s, i.e. 256 CUs x 4 SIMDs x 32 lanes x 2.4 GHz
+# (two wave64 instructions per CU per cycle; rocprofv3 counts the clique
+# tier-0 launch at 1.05 per CU-cycle, above what 16-lane SIMDs could issue).
+VALU_PEAK_TOPS = 256 * 4 * 32 * 2.4e9 / 1e12
+CLIQUE_KERNELS = ("gatk::clique_kernel<0>", "gatk::clique_block_kernel<16, true, 0, 8, 0>",
+                  "gatk::clique_block_kernel<32, true, 1, 16, 0>")
+
+
+def load_profile(name, cfg):
+    """rocprofv3 summary in profiles/ (graphautotune_amd.profsum), when it was
+    taken with this kernel config."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    return prof if prof.get("config") == cfg else None
+
+
+def clique_roofline(kernel, ms, walg, prof):
+    """Bitmap clique store: its operands are bit rows in LDS, its work is
+    integer bit operations on the vector ALU (rocprofv3 counters in the
+    profile give the VALU issue rate, lane use and LDS use beside it).
+    Algorithmic integer operations = W_alg / 4: every 4-byte operand word of
+    an AND of rows is one AND (or one three-input bit op) or one
+    popcount-accumulate."""
+    ops = walg / 4.0
+    achieved = ops / (ms * 1e-3) / 1e12
+    out = {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+           "frac": achieved / VALU_PEAK_TOPS, "traffic": None, "kernel": kernel, "kernel_ms": ms,
+           "alg_bytes_per_launch": walg, "alg_ops_per_launch": ops,
+           "peak_source": "MI355X 157.3 TFLOP/s FP32 vector peak / 2 flops per FMA (256 CUs x 4 SIMDs x 32 "
+                          "lanes x 2.4 GHz)"}
+    if prof:
+        hbm = prof.get("hbm_bytes_per_launch", {}).get(kernel)
+        if hbm:
+            out["traffic"] = hbm
+            out["hbm_achieved_gbps"] = hbm / (ms * 1e-3) / 1e9
+            out["hbm_frac"] = out["hbm_achieved_gbps"] / HBM_PEAK_GBPS
+        d = prof.get("derived", {}).get(kernel)
+        if d:
+            out["counters"] = {k: d[k] for k in ("valu_issue", "valu_lane_util", "lds_util", "lds_conflict_share",
+                                                 "wait_share", "waves_per_cu") if k in d}
+    return out
+
+
+def hbm_roofline(kernel, ms, walg, prof):
+    achieved = walg / (ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": kernel, "kernel_ms": ms,
+           "alg_bytes_per_launch": walg}
+    if prof:
+        hbm = prof.get("hbm_bytes_per_launch", {}).get(kernel)
+        if hbm:
+            out["traffic"] = hbm
+            out["hbm_frac"] = hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS
+    return out
 
 
 def log(*a):
@@ -325,11 +388,15 @@ def main():
     second_ms = statistics.median(mains[1])
     # the roofline is for the launch that dominates the step
     bitmap = cfg.get("vertex_store", 0) != 1
-    launches = [("gatk::clique_kernel" if bitmap else "gatk::engine_kernel<false>", main_ms, walg_main),
-                ("gatk::clique_block_kernel<16, true, 0, 8>" if bitmap else "gatk::engine_kernel<true>", second_ms,
-                 walg_second)]
+    walg_rest = walg - walg_main - walg_second
+    rest_ms = t_b * 1e3 - main_ms - second_ms
+    if bitmap:
+        launches = [(CLIQUE_KERNELS[0], main_ms, walg_main), (CLIQUE_KERNELS[1], second_ms, walg_second),
+                    (CLIQUE_KERNELS[2] + " and tier 2", rest_ms, walg_rest)]
+    else:
+        launches = [("gatk::engine_kernel<false>", main_ms, walg_main),
+                    ("split and HBM launches", second_ms, walg_second)]
     kernel_name, kernel_ms, kernel_walg = max(launches, key=lambda x: x[1])
-    achieved = kernel_walg / (kernel_ms * 1e-3) / 1e9
     t_plain, plain_ms, t_plain_same, plain_same_ms = None, None, None, None
     if plains:
         c4, s4, _, _, _ = timed(plain4, {}, args.steps)
@@ -352,13 +419,11 @@ def main():
         % (counts[0], counts[1], t_b * 1e3, main_ms, second_ms, "%.3f" % (t_plain * 1e3) if t_plain else "-",
            st["overflow_tasks"]))
 
-    traffic = None  # HBM bytes per launch of that kernel, from rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    tpath = os.path.join(ROOT, "profiles", "traffic_rmat%d_k4k5.json" % args.scale)
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tr = json.load(f)
-        if tr.get("config") == cfg:
-            traffic = tr.get("bytes_per_launch", {}).get(kernel_name)
+    prof = load_profile("clique_rmat%d_k4k5.json" % args.scale, cfg)
+    roofline = clique_roofline(kernel_name, kernel_ms, kernel_walg, prof) if bitmap else \
+        hbm_roofline(kernel_name, kernel_ms, kernel_walg, prof)
+    roofline.update({"launches_ms": {name: ms for name, ms, _ in launches}, "step_alg_bytes": walg,
+                     "step_alg_bytes_per_s": walg / t_b})
 
     out = {
         "metric": METRIC,
@@ -390,25 +455,7 @@ def main():
         "batched_vs_plain": t_plain / t_b if t_plain else None,
         "device_ms_plain_same_config": plain_same_ms,
         "batched_vs_plain_same_config": t_plain_same / t_b if t_plain_same else None,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": traffic,
-            # measured HBM rate of the same launch: far below `achieved` when
-            # the operands are read from LDS (the bitmap rows of the
-            # workgroup tiers), so `frac` can exceed 1 there
-            "hbm_achieved": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
-            "hbm_frac": traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None,
-            "alg_bytes_per_launch": kernel_walg,
-            "kernel": kernel_name,
-            "kernel_ms": kernel_ms,
-            "launches_ms": {name: ms for name, ms, _ in launches},
-            "step_alg_bytes": walg,
-            "step_achieved": walg / t_b / 1e9,
-        },
+        "roofline": roofline,
     }
     if args.timeline:
         _, tst = batched.count(cfg, timeline=True)

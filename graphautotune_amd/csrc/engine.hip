@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "handles.hpp"
@@ -1262,6 +1263,10 @@ __device__ __forceinline__ int clique_parts(int tier, int d) {
 }
 constexpr int CBLOCKW0 = 8;   // waves per workgroup, tier 0
 constexpr int CBLOCKW1 = 16;  // waves per workgroup, tiers 1 and 2 (few, heavy vertices)
+// Words per bit row of a list of d vertices: enough for d bits, rounded up
+// to an odd number so that lanes reading the same word of different rows
+// hit different LDS banks (an even stride folds them onto a few banks).
+__host__ __device__ constexpr int row_words(int d) { return ((d + 31) >> 5) | 1; }
 
 struct CArgs {
     const long long *off;
@@ -1297,7 +1302,7 @@ struct CArgs {
 // lanes sweep the (row, element) pairs of 64 of these rows at a time; each
 // element of N+(L[p]) is searched in L[0,p), or each L[q], q < p, in
 // N+(L[p]) when that list is gallop_ratio times longer than p.
-template <class LP, class RP>
+template <bool ACC, class LP, class RP>
 __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first, int stride, int ln,
                          unsigned long long &bytes) {
     for (int cb = 0; first + cb * stride < d; cb += W) {
@@ -1308,7 +1313,7 @@ __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first,
         const int xl = valid ? a.split[x] : 0;  // N+(x): all below x
         const bool sx = (long long)xl <= (long long)a.gallop_ratio * (long long)ci;
         const int mylen = valid ? (sx ? xl : ci) : 0;
-        if (a.account && valid) bytes += 4ull * (xl + ci) + 16ull;
+        if (ACC && valid) bytes += 4ull * (xl + ci) + 16ull;
         int T = 0;
         const int myoff = excl_scan(mylen, &T);
         for (int k0 = 0; k0 < T; k0 += W) {
@@ -1372,6 +1377,39 @@ __device__ __forceinline__ unsigned pick(const unsigned (&t)[N], int i) {
     return v;
 }
 
+// sum over TC <= u < n of popc(a[u] & b[u]) (and of a[u] & b[u] & c[u]):
+// with MAXW > 0 (n <= MAXW) the loop is unrolled, so every word is read at
+// an immediate offset from its row's base and costs its reads, one bit op
+// and one popcount; MAXW = 0 loops (rows of any width).
+template <int MAXW, int TC, class RP>
+__device__ __forceinline__ unsigned and2_popc(RP a, RP b, int n) {
+    unsigned c = 0;
+    if constexpr (MAXW > 0) {
+#pragma unroll
+        for (int u = TC; u < MAXW; u++) {
+            if (u >= n) break;
+            c += __popc(a[u] & b[u]);
+        }
+    } else {
+        for (int u = TC; u < n; u++) c += __popc(a[u] & b[u]);
+    }
+    return c;
+}
+template <int MAXW, int TC, class RP>
+__device__ __forceinline__ unsigned and3_popc(RP a, RP b, RP c3, int n) {
+    unsigned c = 0;
+    if constexpr (MAXW > 0) {
+#pragma unroll
+        for (int u = TC; u < MAXW; u++) {
+            if (u >= n) break;
+            c += __popc(a[u] & b[u] & c3[u]);
+        }
+    } else {
+        for (int u = TC; u < n; u++) c += __popc(a[u] & b[u] & c3[u]);
+    }
+    return c;
+}
+
 // Counts at and below the pair level. The pairs (p, q), q in M_p, are
 // numbered through off[p] = sum of popc(M_p') over p' < p (off[d] = P), and
 // worker k takes pairs first, first + step, ..., so lanes share the work
@@ -1379,8 +1417,10 @@ __device__ __forceinline__ unsigned pick(const unsigned (&t)[N], int i) {
 // += popc(t); each r in t adds popc(t & M_r) to #K5; each s in t & M_r adds
 // popc(t & M_r & M_s) to #K6. Words are read from the rows as needed (no
 // per-lane row copies: that keeps register use, and so occupancy, low),
-// each loop stopping at the last word that can hold bits.
-template <class RP>
+// each loop stopping at the last word that can hold bits. Rows are w words
+// apart (w <= MAXW when MAXW > 0); row offsets fit 32 bits and their
+// factors 24 bits, so they take full-rate 24-bit multiplies.
+template <int MAXW, bool ACC, class RP>
 __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off, int P, int first, int step,
                           unsigned long long *acc, unsigned long long &bytes) {
     for (int k = first; k < P; k += step) {
@@ -1390,10 +1430,11 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
             if (off[mid] <= k) lo = mid; else hi = mid;
         }
         const int p = lo;
+        const RP rp = rows + __umul24(p, w);
         int j = k - off[p];
         int q = 0;
         for (int u = 0; u <= (p >> 5); u++) {
-            const unsigned x = rows[(long long)p * w + u];
+            const unsigned x = rp[u];
             const int c = __popc(x);
             if (j < c) {
                 q = u * 32 + select_bit(x, j);
@@ -1402,8 +1443,8 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
             j -= c;
         }
         const int wq = (q >> 5) + 1;
-        if (a.account) bytes += 8ull * wq;
-        const long long pw = (long long)p * w, qw = (long long)q * w;
+        if (ACC) bytes += 8ull * wq;
+        const RP rq = rows + __umul24(q, w);
         // the first TC words of t = M_p & M_q stay in registers; the rest
         // are re-read from the rows when needed
         constexpr int TC = 2;
@@ -1411,58 +1452,66 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
         unsigned c4 = 0;
 #pragma unroll
         for (int u = 0; u < TC; u++) {
-            tc[u] = u < wq ? (rows[pw + u] & rows[qw + u]) : 0u;
+            tc[u] = u < wq ? (rp[u] & rq[u]) : 0u;
             c4 += __popc(tc[u]);
         }
-        for (int u = TC; u < wq; u++) c4 += __popc(rows[pw + u] & rows[qw + u]);
+        c4 += and2_popc<MAXW, TC>(rp, rq, wq);
         acc[4] += c4;
         if (a.kmax < 5) continue;
+        unsigned c5 = 0;  // at most |t| * d bits: fits 32 bits
         for (int rw = 0; rw < wq; rw++) {
-            unsigned br = rw < TC ? pick(tc, rw) : (rows[pw + rw] & rows[qw + rw]);
+            // every r in word rw of t needs words 0 .. rw (the same count
+            // for all lanes still in the loop)
+            unsigned br = rw < TC ? pick(tc, rw) : (rp[rw] & rq[rw]);
             while (br) {
                 const int r = rw * 32 + __ffs(br) - 1;
                 br &= br - 1;
-                const int wr = (r >> 5) + 1;
-                const long long rr = (long long)r * w;
-                if (a.account) bytes += 8ull * wr;
-                unsigned c5 = 0;
-#pragma unroll
-                for (int u = 0; u < TC; u++)
-                    if (u < wr) c5 += __popc(tc[u] & rows[rr + u]);
-                for (int u = TC; u < wr; u++) c5 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u]);
-                acc[5] += c5;
+                const RP rr = rows + __umul24(r, w);
+                if (ACC) bytes += 8ull * (rw + 1);
+                // words 0 and 1 of t sit in registers. Word 1 of M_r is read
+                // whatever rw is: it is 0 when r < 32, and it lies within the
+                // rows (lists of more than 32 vertices have rows of 3 or more
+                // words; a shorter list has t's word 1 = 0, and its slice of
+                // at least 32 words runs past row r + 1)
+                c5 += __popc(tc[0] & rr[0]) + __popc(tc[1] & rr[1]);
+                if (rw >= TC) c5 += and3_popc<MAXW, TC>(rp, rq, rr, rw + 1);
                 if (a.kmax < 6) continue;
                 unsigned c6 = 0;
-                for (int sw = 0; sw < wr; sw++) {
-                    unsigned bs = rows[pw + sw] & rows[qw + sw] & rows[rr + sw];
+                for (int sw = 0; sw <= rw; sw++) {
+                    unsigned bs = rp[sw] & rq[sw] & rr[sw];
                     while (bs) {
                         const int sx = sw * 32 + __ffs(bs) - 1;
                         bs &= bs - 1;
-                        const long long ss = (long long)sx * w;
-                        if (a.account) bytes += 8ull * ((sx >> 5) + 1);
-                        for (int u = 0; u <= (sx >> 5); u++)
-                            c6 += __popc(rows[pw + u] & rows[qw + u] & rows[rr + u] & rows[ss + u]);
+                        const RP rs = rows + __umul24(sx, w);
+                        if (ACC) bytes += 8ull * ((sx >> 5) + 1);
+                        for (int u = 0; u <= (sx >> 5); u++) c6 += __popc(rp[u] & rq[u] & rr[u] & rs[u]);
                     }
                 }
                 acc[6] += c6;
             }
         }
+        acc[5] += c5;
     }
 }
 
+template <bool ACC>
 __device__ __forceinline__ void clique_flush(const CArgs &a, unsigned long long *acc, unsigned long long bytes,
                                              int ln) {
     for (int k = 3; k <= 6; k++) acc[k] = wave_sum64(acc[k]);
-    bytes = wave_sum64(bytes);
+    if (ACC) bytes = wave_sum64(bytes);
     if (ln == 0) {
         for (int k = 3; k <= 6; k++)
             if (a.pat_of_k[k] >= 0 && acc[k]) atomicAdd(&a.counts[a.pat_of_k[k]], acc[k]);
-        if (a.account && bytes) atomicAdd(a.bytes, bytes);
+        if (ACC && bytes) atomicAdd(a.bytes, bytes);
     }
 }
 
 // Main tier: one wave per vertex, vertices from the main queue, d <= a.cap
 // (rows in LDS, 8 words wide); larger vertices go to the block tiers.
+// MODE bit 0: accumulate W_alg; bit 1: timeline records (separate
+// instantiations, so that a timed count carries none of their work or
+// registers).
+template <int MODE>
 __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
@@ -1470,8 +1519,8 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
     int *L = reinterpret_cast<int *>(base);
     unsigned *rows = reinterpret_cast<unsigned *>(L + a.cap + 4);  // L: d entries, then d + 1 offsets
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;  // per lane
-    const bool tl = a.tl != nullptr;
-    const long long tl_t0 = wall_clock64();
+    constexpr bool ACC = MODE & 1, tl = (MODE & 2) != 0;
+    const long long tl_t0 = tl ? wall_clock64() : 0;
     long long tl_build = 0, tl_count = 0, tl_items = 0, tl_pairs = 0;
     for (;;) {
         unsigned long long r = 0;
@@ -1499,12 +1548,12 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             }
             const long long c0 = tl ? clock64() : 0;
             const long long rb = a.off[v0];
-            const int w = (d + 31) >> 5;
+            const int w = row_words(d);
             for (int k = ln; k < d; k += W) L[k] = a.adj[rb + k];
             for (int k = ln; k < d * w; k += W) rows[k] = 0u;
-            if (a.account && ln == 0) bytes += 4ull * d + 16ull;
+            if (ACC && ln == 0) bytes += 4ull * d + 16ull;
             wsync();
-            vtx_rows(a, L, d, rows, w, 0, 1, ln, bytes);
+            vtx_rows<ACC>(a, L, d, rows, w, 0, 1, ln, bytes);
             wsync();
             // L is no longer needed: it holds the pair offsets from here on
             int *off = L;
@@ -1512,7 +1561,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             for (int pb = 0; pb < d; pb += W) {
                 const int p = pb + ln;
                 const int c = p < d ? row_popc(rows, p, w) : 0;
-                if (a.account && p < d) bytes += 4ull * ((p >> 5) + 1);
+                if (ACC && p < d) bytes += 4ull * ((p >> 5) + 1);
                 int tot = 0;
                 const int x = excl_scan(c, &tot);
                 if (p < d) off[p] = run + x;
@@ -1524,7 +1573,8 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             }
             wsync();
             const long long c1 = tl ? clock64() : 0;
-            if (a.kmax >= 4) vtx_pairs(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
+            if (a.kmax >= 4)
+                vtx_pairs<row_words(32 * CMAXW), ACC>(a, (const unsigned *)rows, d, w, off, run, ln, W, acc, bytes);
             wsync();
             if (tl) {
                 tl_build += c1 - c0;
@@ -1534,7 +1584,7 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
             }
         }
     }
-    clique_flush(a, acc, bytes, ln);
+    clique_flush<ACC>(a, acc, bytes, ln);
     if (tl && ln == 0) {
         const int unit = blockIdx.x * (blockDim.x / W) + wid;
         tl_write(a.tl, a.tl_base + unit, 0, unit, tl_t0, wall_clock64(), tl_build, tl_count, tl_items, tl_pairs);
@@ -1546,8 +1596,8 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
 // in the workgroup's HBM scratch; MW == 0 (any d up to hcap) reads every
 // word from HBM as needed. The waves split the rows between them for the
 // build and take pairs round-robin for the count.
-template <int MW, bool LDS, int TIER, int NW>
-__global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
+template <int MW, bool LDS, int TIER, int NW, int MODE>
+__global__ void __launch_bounds__(64 * NW, NW == CBLOCKW0 ? 8 : 1) clique_block_kernel(CArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wid = threadIdx.x / W, ln = threadIdx.x % W;
     constexpr int lcap = 32 * MW;
@@ -1556,12 +1606,12 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
     unsigned *rows_l = reinterpret_cast<unsigned *>(Ls + (LDS ? lcap + 4 : 0));
     __shared__ long long vsh;
     __shared__ int wtot[NW];
-    const int hws = (a.hcap + 31) / 32;
-    unsigned *rows_g = a.grows + (long long)blockIdx.x * a.hcap * hws;
-    int *off_g = a.goff + (long long)blockIdx.x * (a.hcap + 1);
+    const int hws = row_words(a.hcap);
+    unsigned *rows_g = LDS ? nullptr : a.grows + (long long)blockIdx.x * a.hcap * hws;
+    int *off_g = LDS ? nullptr : a.goff + (long long)blockIdx.x * (a.hcap + 1);
     unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0}, bytes = 0;
-    const bool tl = a.tl != nullptr;
-    const long long tl_t0 = wall_clock64();
+    constexpr bool ACC = MODE & 1, tl = (MODE & 2) != 0;
+    const long long tl_t0 = tl ? wall_clock64() : 0;
     long long tl_build = 0, tl_count = 0, tl_items = 0, tl_pairs = 0, c0 = 0, c1 = 0;
     for (;;) {
         if (tl) c0 = clock64();
@@ -1577,7 +1627,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
         const int d = a.split[v0];
         const int parts = clique_parts(TIER, d);
         const long long rb = a.off[v0];
-        const int w = (d + 31) >> 5;
+        const int w = row_words(d);
         if (d > (MW > 0 ? lcap : a.hcap) || d > a.hcap) {  // cannot happen: lists are split by d
             if (threadIdx.x == 0) atomicOr(a.err, 8u);
         } else {
@@ -1587,11 +1637,11 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             for (long long k = threadIdx.x; k < (long long)d * w; k += 64 * NW) rows[k] = 0u;
             // every part builds the rows; W_alg charges one build
             unsigned long long build_bytes = 0;
-            if (a.account && threadIdx.x == 0) build_bytes += 4ull * d + 16ull;
+            if (ACC && threadIdx.x == 0) build_bytes += 4ull * d + 16ull;
             if (!LDS) __threadfence();
             __syncthreads();
             const int *L = LDS ? Ls : a.adj + rb;
-            vtx_rows(a, L, d, rows, w, wid, NW, ln, build_bytes);
+            vtx_rows<ACC>(a, L, d, rows, w, wid, NW, ln, build_bytes);
             if (!LDS) __threadfence();
             __syncthreads();
             // pair offsets: block-wide exclusive prefix of popc(M_p)
@@ -1600,7 +1650,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             for (int pb = 0; pb < d; pb += 64 * NW) {
                 const int p = pb + (int)threadIdx.x;
                 const int c = p < d ? row_popc((const unsigned *)rows, p, w) : 0;
-                if (a.account && p < d) build_bytes += 4ull * ((p >> 5) + 1);
+                if (ACC && p < d) build_bytes += 4ull * ((p >> 5) + 1);
                 int tot = 0;
                 const int x = excl_scan(c, &tot);
                 if (ln == 0) wtot[wid] = tot;
@@ -1624,7 +1674,8 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             __syncthreads();
             if (tl) c1 = clock64();
             if (a.kmax >= 4)
-                vtx_pairs(a, (const unsigned *)rows, d, w, off, run, part * 64 * NW + (int)threadIdx.x,
+                vtx_pairs<(MW > 0 ? row_words(32 * MW) : 0), ACC>(a, (const unsigned *)rows, d, w, off, run,
+                                                           part * 64 * NW + (int)threadIdx.x,
                           64 * NW * parts, acc, bytes);
             if (tl) {
                 tl_pairs += run;
@@ -1638,7 +1689,7 @@ __global__ void __launch_bounds__(64 * NW) clique_block_kernel(CArgs a) {
             c1 = 0;
         }
     }
-    clique_flush(a, acc, bytes, ln);
+    clique_flush<ACC>(a, acc, bytes, ln);
     if (tl && threadIdx.x == 0)
         tl_write(a.tl, a.tl_base + blockIdx.x, 1 + TIER, blockIdx.x, tl_t0, wall_clock64(), tl_build, tl_count,
                  tl_items, tl_pairs);
@@ -1927,6 +1978,30 @@ static void fill_defaults(gat_kernel_config *c) {
     if (c->split_us == 0) c->split_us = kSplitUs;
 }
 
+// Knobs and launch shapes of the clique store: the main launch (one wave
+// per vertex, rows in the wave's LDS slice) and workgroup tiers 0 and 1
+// (rows in LDS; t1_lds = 0 when the device cannot give tier 1 its LDS).
+static int clique_shape(const DeviceInfo &di, const gat_kernel_config &cfg, Launch *L) {
+    gatk::CArgs &c = L->ca;
+    c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
+    c.chunk = cfg.queue_chunk;
+    c.lin_thr = cfg.search_linear_threshold;
+    c.gallop_ratio = cfg.gallop_ratio;
+    L->block = dim3(64 * cfg.block_waves);
+    L->grid = dim3(std::max(1, std::max(1, cfg.waves_per_cu * di.cus) / cfg.block_waves));
+    L->clique_wave_bytes = (4 * (c.cap + 4) + 4 * c.cap * gatk::row_words(c.cap) + 15) / 16 * 16;
+    L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
+    if (L->clique_lds > di.lds_per_block)
+        GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
+    L->mid_lds = (size_t)4 * (32 * gatk::CMIDW + 4) + (size_t)4 * 32 * gatk::CMIDW * gatk::row_words(32 * gatk::CMIDW);
+    if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique block tier");
+    L->mid_grid = dim3(std::max(1, (int)(di.lds_per_cu / L->mid_lds)) * di.cus);
+    const size_t t1 = (size_t)4 * (32 * gatk::CBIGW + 4) + (size_t)4 * 32 * gatk::CBIGW * gatk::row_words(32 * gatk::CBIGW);
+    L->t1_lds = t1 <= di.lds_per_block ? t1 : 0;
+    if (L->t1_lds) L->t1_grid = dim3(std::max(1, (int)(di.lds_per_cu / t1)) * di.cus);
+    return GAT_OK;
+}
+
 static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     DeviceInfo di;
     int rc = device_info(&di);
@@ -2065,10 +2140,6 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         c.ovf_cap = (long long)(g.n + 1) * gatk::CSPLIT_ENC;
         c.bytes = a.bytes;
         c.err = a.err;
-        c.cap = std::max(32, std::min(32 * gatk::CMAXW, cfg.slot_capacity / 32 * 32));
-        c.chunk = cfg.queue_chunk;
-        c.lin_thr = cfg.search_linear_threshold;
-        c.gallop_ratio = cfg.gallop_ratio;
         // HBM tier: rows sized by the maximum out-degree
         if (g.max_out < 0) {
             int32_t mo = 0;
@@ -2079,7 +2150,7 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
             g.max_out = mo;
         }
         const int hcap = std::max(32, g.max_out);
-        const int ws = (hcap + 31) / 32;
+        const int ws = gatk::row_words(hcap);
         int waves = std::max(64, 8 * di.cus);
         while (waves > 64 && (double)waves * hcap * ws * 4 > 16e9) waves /= 2;
         if (d->big_cap < hcap || d->big_waves < waves) {
@@ -2096,17 +2167,10 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
         c.goff = d->goff;
         c.hcap = d->big_cap;
         L->ca = c;
-        L->clique_wave_bytes = (4 * (c.cap + 4) + 4 * c.cap * (c.cap / 32) + 15) / 16 * 16;
-        L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
-        if (L->clique_lds > di.lds_per_block)
-            GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
-        L->mid_lds = (size_t)4 * (32 * gatk::CMIDW + 4) + (size_t)4 * 32 * gatk::CMIDW * gatk::CMIDW;
-        if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique block tier");
-        L->mid_grid = dim3(std::max(1, (int)(di.lds_per_cu / L->mid_lds)) * di.cus);
+        rc = clique_shape(di, cfg, L);
+        if (rc) return rc;
         L->big_grid = dim3(std::min(d->big_waves, 2 * di.cus));  // one scratch area per workgroup
-        const size_t t1 = (size_t)4 * (32 * gatk::CBIGW + 4) + (size_t)4 * 32 * gatk::CBIGW * gatk::CBIGW;
-        L->t1_lds = t1 <= di.lds_per_block ? t1 : 0;
-        L->t1_grid = L->t1_lds ? dim3(std::max(1, (int)(di.lds_per_cu / t1)) * di.cus) : L->big_grid;
+        if (!L->t1_lds) L->t1_grid = L->big_grid;
     }
     set_roots(L, g, 0, g.n);
     return GAT_OK;
@@ -2150,32 +2214,42 @@ static int enqueue(PlanDevice *d, const Launch &L, bool account, bool events, bo
     c.account = account ? 1 : 0;
     c.tl = tl;
     if (L.clique) {
-        c.tl_base = tl_take(waves);
-        hipLaunchKernelGGL(gatk::clique_kernel, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
+        // W_alg accounting and timeline records come from separate kernel
+        // instantiations (MODE bits 0 and 1)
+        auto run = [&](auto mode_tag) -> int {
+            constexpr int T = decltype(mode_tag)::value;
+            c.tl_base = tl_take(waves);
+            hipLaunchKernelGGL(gatk::clique_kernel<T>, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
+            if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
+            c.bytes = d->bytes + 1;  // workgroup tier 0: its own W_alg counter
+            c.tl_base = tl_take((int)L.mid_grid.x);
+            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW0, T>), L.mid_grid,
+                               dim3(64 * gatk::CBLOCKW0), L.mid_lds, s, c);
+            if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
+            c.bytes = d->bytes + 2;  // tiers 1 and 2
+            c.tl_base = tl_take((int)L.t1_grid.x);
+            if (L.t1_lds)
+                hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW1, T>), L.t1_grid,
+                                   dim3(64 * gatk::CBLOCKW1), L.t1_lds, s, c);
+            else
+                hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW1, T>), L.t1_grid,
+                                   dim3(64 * gatk::CBLOCKW1), 0, s, c);
+            c.tl_base = tl_take((int)L.big_grid.x);
+            hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1, T>), L.big_grid,
+                               dim3(64 * gatk::CBLOCKW1), 0, s, c);
+            return GAT_OK;
+        };
+        const int mode = (account ? 1 : 0) | (timeline ? 2 : 0);
+        const int rc = mode == 0   ? run(std::integral_constant<int, 0>{})
+                       : mode == 1 ? run(std::integral_constant<int, 1>{})
+                       : mode == 2 ? run(std::integral_constant<int, 2>{})
+                                   : run(std::integral_constant<int, 3>{});
+        if (rc) return rc;
     } else {
         a.tl_base = tl_take(waves);
         hipLaunchKernelGGL(gatk::put_kargs, dim3(1), dim3(64), 0, s, a, d->dargs);
         hipLaunchKernelGGL(gatk::engine_kernel<false>, L.grid, L.block, L.lds, s, d->dargs, L.wave_bytes);
-    }
-    if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
-    if (L.clique) {
-        c.bytes = d->bytes + 1;  // workgroup tier 0: its own W_alg counter
-        c.tl_base = tl_take((int)L.mid_grid.x);
-        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW0>), L.mid_grid,
-                           dim3(64 * gatk::CBLOCKW0), L.mid_lds, s, c);
-        if (events) HIPCHECK(hipEventRecord(d->ev[3], s));
-        c.bytes = d->bytes + 2;  // tiers 1 and 2
-        c.tl_base = tl_take((int)L.t1_grid.x);
-        if (L.t1_lds)
-            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW1>), L.t1_grid,
-                               dim3(64 * gatk::CBLOCKW1), L.t1_lds, s, c);
-        else
-            hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, false, 1, gatk::CBLOCKW1>), L.t1_grid,
-                               dim3(64 * gatk::CBLOCKW1), 0, s, c);
-        c.tl_base = tl_take((int)L.big_grid.x);
-        hipLaunchKernelGGL((gatk::clique_block_kernel<0, false, 2, gatk::CBLOCKW1>), L.big_grid,
-                           dim3(64 * gatk::CBLOCKW1), 0, s, c);
-    } else {
+        if (events) HIPCHECK(hipEventRecord(d->ev[1], s));
         if (a.split_ticks > 0)  // split subtasks, level by level, in LDS like the main launch
             for (int k = 1; k <= kLevels; k++) {
                 gatk::KArgs sk = a;
