@@ -396,6 +396,18 @@ def default_config():
     return c
 
 
+def clique_counts_noalloc(d_offsets, d_adj, v_cnt, kmax, config=None, stream=None):
+    """#K3 .. #K_kmax of a CSR graph already in device memory (device
+    addresses as ints) through the allocation-free entry point
+    gat_count_cliques_noalloc; `stream` is a hipStream_t address or None."""
+    n = int(kmax) - 2
+    out = (C.c_uint64 * max(1, n))()
+    cfg = config if isinstance(config, KernelConfig) else KernelConfig.from_dict(config or {})
+    check(lib().gat_count_cliques_noalloc(C.c_void_p(int(d_offsets)), C.c_void_p(int(d_adj)), int(v_cnt), int(kmax),
+                                          C.byref(cfg), C.c_void_p(stream) if stream else None, out))
+    return [int(x) for x in out[:n]]
+
+
 def device_count():
     n = C.c_int32()
     lib().gat_device_count(C.byref(n))

@@ -1537,11 +1537,15 @@ __global__ void __launch_bounds__(256) clique_kernel(CArgs a, int wave_bytes) {
                     const int t = d <= 32 * CMIDW ? 0 : (d <= 32 * CBIGW ? 1 : 2);
                     const int parts = clique_parts(t, d);
                     atomicAdd(&a.ovf_n[3], 1ull);  // vertices handed on (statistics)
-                    const unsigned long long idx = atomicAdd(&a.ovf_n[t], (unsigned long long)parts);
-                    if ((long long)(idx + parts) <= a.ovf_cap) {
-                        for (int k = 0; k < parts; k++) a.tier[t][idx + k] = (long long)v0 * CSPLIT_ENC + k;
+                    if (t == 2 && !a.grows) {
+                        atomicOr(a.err, 16u);  // no HBM tier in this count (allocation-free entry point)
                     } else {
-                        atomicOr(a.err, 4u);
+                        const unsigned long long idx = atomicAdd(&a.ovf_n[t], (unsigned long long)parts);
+                        if ((long long)(idx + parts) <= a.ovf_cap) {
+                            for (int k = 0; k < parts; k++) a.tier[t][idx + k] = (long long)v0 * CSPLIT_ENC + k;
+                        } else {
+                            atomicOr(a.err, 4u);
+                        }
                     }
                 }
                 continue;
@@ -2412,6 +2416,113 @@ extern "C" int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32
     rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true, (flags & GAT_FLAG_TIMELINE) != 0);
     if (rc) return rc;
     return finish(p, L, raw_out, stats, true, true);
+}
+
+// ---- allocation-free clique count ---------------------------------------
+// Scratch of gat_count_cliques_noalloc, one copy per device for the life of
+// the process: kNaHead words of counters (counts [0,8), queues [8,24),
+// work-list sizes [24,40), W_alg [40,44), error flags at word 44), then the
+// split[] array (int32, kNaMaxV entries), then the lists of workgroup tiers 0
+// and 1 (int64, kNaTierCap entries each): 32 MiB and 512 B in all.
+constexpr int kNaMaxV = 1 << 22;
+constexpr long long kNaTierCap = 1 << 20;
+constexpr long long kNaHead = 64;
+constexpr long long kNaWords = kNaHead + kNaMaxV / 2 + 2 * kNaTierCap;
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+
+// Device address of this device's scratch, looked up once per device (the
+// host-side name `scratch` is only a handle for the runtime, never an address).
+static int noalloc_scratch(unsigned long long **out) {
+    static std::mutex mu;
+    static unsigned long long *addr[64] = {};
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) GAT_FAIL(GAT_ENODEV, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            GAT_FAIL(GAT_ENODEV, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        addr[dev] = static_cast<unsigned long long *>(ptr);
+    }
+    *out = addr[dev];
+    return GAT_OK;
+}
+
+extern "C" int gat_count_cliques_noalloc(const int64_t *d_offsets, const int32_t *d_adj, int32_t v_cnt, int32_t kmax,
+                                         const gat_kernel_config *cfg_in, void *stream, uint64_t *counts_out) {
+    if (!d_offsets || !d_adj || !counts_out) GAT_FAIL(GAT_EINVAL, "null graph or counts pointer");
+    if (kmax < 3 || kmax > 6) GAT_FAIL(GAT_EINVAL, "kmax %d is not within [3, 6]", kmax);
+    if (v_cnt < 0) GAT_FAIL(GAT_EINVAL, "negative v_cnt");
+    if (v_cnt > kNaMaxV)
+        GAT_FAIL(GAT_ELIMIT, "v_cnt %d exceeds the allocation-free count's limit of %d vertices", v_cnt, kNaMaxV);
+    DeviceInfo di;
+    int rc = device_info(&di);
+    if (rc) return rc;
+    gat_kernel_config cfg{};
+    if (cfg_in) cfg = *cfg_in;
+    fill_defaults(&cfg);
+    if (cfg.block_waves > 4) GAT_FAIL(GAT_EINFEASIBLE, "block_waves %d > 4", cfg.block_waves);
+    Launch L;
+    rc = clique_shape(di, cfg, &L);
+    if (rc) return rc;
+    if (!L.t1_lds)
+        GAT_FAIL(GAT_EINFEASIBLE, "the allocation-free count needs %d KiB of LDS per workgroup",
+                 (int)((4 * (32 * gatk::CBIGW + 4) + 4 * 32 * gatk::CBIGW * gatk::row_words(32 * gatk::CBIGW)) / 1024));
+    unsigned long long *h = nullptr;
+    rc = noalloc_scratch(&h);
+    if (rc) return rc;
+    gatk::CArgs &c = L.ca;
+    c.off = reinterpret_cast<const long long *>(d_offsets);
+    c.adj = d_adj;
+    int *split = reinterpret_cast<int *>(h + kNaHead);
+    c.split = split;
+    c.n = v_cnt;
+    c.m = 0;
+    c.root_lo = 0;
+    c.root_hi = v_cnt;
+    c.counts = h;
+    for (int k = 0; k < 8; k++) c.pat_of_k[k] = (k >= 3 && k <= kmax) ? k - 3 : -1;
+    c.kmax = kmax;
+    c.queue = h + 8;
+    c.tier[0] = reinterpret_cast<long long *>(h + kNaHead + kNaMaxV / 2);
+    c.tier[1] = c.tier[0] + kNaTierCap;
+    c.tier[2] = nullptr;
+    c.ovf_n = h + 24;
+    c.ovf_cap = kNaTierCap;
+    c.bytes = h + 40;
+    c.err = reinterpret_cast<unsigned *>(h + 44);
+    c.grows = nullptr;  // no HBM tier: vertices with more than 32 * CBIGW neighbours below them are refused
+    c.goff = nullptr;
+    c.hcap = 32 * gatk::CBIGW;
+    c.account = 0;
+    c.tl = nullptr;
+    c.tl_base = 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(gatk::fill_u64, dim3(1), dim3(64), 0, s, h, kNaHead, 0ull);
+    if (v_cnt > 0) {
+        hipLaunchKernelGGL(gatk::split_kernel, dim3(1024), dim3(256), 0, s, reinterpret_cast<const long long *>(d_offsets),
+                           d_adj, v_cnt, split);
+        hipLaunchKernelGGL(gatk::clique_kernel<0>, L.grid, L.block, L.clique_lds, s, c, L.clique_wave_bytes);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CMIDW, true, 0, gatk::CBLOCKW0, 0>), L.mid_grid,
+                           dim3(64 * gatk::CBLOCKW0), L.mid_lds, s, c);
+        hipLaunchKernelGGL((gatk::clique_block_kernel<gatk::CBIGW, true, 1, gatk::CBLOCKW1, 0>), L.t1_grid,
+                           dim3(64 * gatk::CBLOCKW1), L.t1_lds, s, c);
+    }
+    HIPCHECK(hipGetLastError());
+    unsigned long long raw[8] = {};
+    unsigned int err = 0;
+    HIPCHECK(hipMemcpyAsync(raw, h, sizeof(raw), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&err, h + 44, sizeof(err), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (err & 16u)
+        GAT_FAIL(GAT_ELIMIT, "a vertex has more than %d neighbours below it; the allocation-free count has no HBM tier",
+                 32 * gatk::CBIGW);
+    if (err & 4u) GAT_FAIL(GAT_ELIMIT, "more than %lld heavy vertices for one workgroup tier", kNaTierCap);
+    if (err) GAT_FAIL(GAT_EINTERNAL, "engine reported an internal inconsistency (flags 0x%x)", err);
+    for (int k = 3; k <= kmax; k++) counts_out[k - 3] = raw[k - 3];
+    return GAT_OK;
 }
 
 extern "C" int gat_count_graph(gat_plan *p, const gat_kernel_config *cfg, int32_t replays, uint64_t *counts_out,

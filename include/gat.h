@@ -323,6 +323,27 @@ int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
                     gat_count_stats *stats);
 void gat_plan_free(gat_plan *p);
 
+/* Allocation-free clique count: #K3 .. #K_kmax (kmax in [3, 6]) of a graph
+ * already in device memory, written to counts_out[0 .. kmax-3] (host). It
+ * runs the kernels of gat_count's bitmap clique store (the vertex of each
+ * clique with the highest id is its root; ids in degree-descending order
+ * keep the per-vertex lists short) and calls no allocating routine: its
+ * counters, per-vertex split array and workgroup-tier lists sit in a static
+ * 32 MiB device array, one per GPU, held until the process ends.
+ *   d_offsets[v_cnt+1], d_adj: device CSR, rows sorted, symmetric, no
+ *   self-loops or duplicates, ids in [0, v_cnt).
+ *   stream: a hipStream_t (NULL = the null stream); the call enqueues on it
+ *   and waits for it.
+ * Limits (GAT_ELIMIT beyond them): v_cnt <= 4,194,304; at most 1,024
+ * neighbours below any vertex (no HBM tier); at most 2^20 vertices with more
+ * than 512 of them. Calls that overlap on one GPU share its scratch array
+ * and race. No counterpart in the reference (engine.cuh:157-216 allocates
+ * its storages per Executor): it is for callers that cannot allocate
+ * device memory, such as code under HIP graph capture or a serving loop. */
+int gat_count_cliques_noalloc(const int64_t *d_offsets, const int32_t *d_adj, int32_t v_cnt,
+                              int32_t kmax, const gat_kernel_config *cfg, void *stream,
+                              uint64_t *counts_out);
+
 /* Records of the last gat_count run with GAT_FLAG_TIMELINE on this plan, 8
  * int64 each: launch (clique store: 0 main, 1-3 workgroup tiers; array
  * store: 0 main, 1-3 subtask levels, 4 HBM launch, 5 its subtasks), unit
