@@ -83,7 +83,11 @@ def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=
     are counted with gat_count_shard, or all of them when there are no
     more. Per stratum h with N_h blocks, n_h sampled, block sums y: total
     N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) / n_h (exact when
-    n_h = N_h). Returns {"raw": [...], "raw_se": [...], "seconds",
+    n_h = N_h). Device time projects the same way, as a sum of block
+    times; that holds when each timed block has enough roots to fill the GPU
+    (a launch of a few light roots costs its fixed launch time, about a
+    millisecond, whatever its work), hence few, large blocks (max_units of
+    64 or so) for time. Returns {"raw": [...], "raw_se": [...], "seconds",
     "seconds_se", "strata": [...]} (raw sums are not divided by the
     divisor)."""
     import numpy as np
@@ -150,6 +154,9 @@ def main(argv=None):
                     help="project the whole-graph counts and device time from a stratified sample of root "
                          "blocks (see stratified_estimate) instead of counting every root")
     ap.add_argument("--per-stratum", type=int, default=24, help="with --estimate: blocks sampled per stratum")
+    ap.add_argument("--max-units", type=int, default=4096,
+                    help="with --estimate: blocks a stratum is cut into (fewer, larger blocks make each timed "
+                         "launch long enough to fill the GPU, so their times add up to the whole count's)")
     ap.add_argument("--seed", type=int, default=0, help="with --estimate: sampling seed")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="count only roots [LO, HI) and report raw sums (add the raw sums of a partition, "
@@ -166,12 +173,13 @@ def main(argv=None):
     plan = core.Plan(g, pats, args.mode)
     cfg = json.loads(args.config) if args.config else {}
     if args.estimate:
-        est = stratified_estimate(plan, cfg, per_stratum=args.per_stratum, seed=args.seed, log=log)
+        est = stratified_estimate(plan, cfg, max_units=args.max_units, per_stratum=args.per_stratum, seed=args.seed,
+                                  log=log)
         infos = [plan.count_info(i) for i in range(len(pats))]
         E = g.undirected_edges()
         print(json.dumps({
             "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
-            "kernel_config": cfg, "per_stratum": args.per_stratum, "seed": args.seed,
+            "kernel_config": cfg, "per_stratum": args.per_stratum, "max_units": args.max_units, "seed": args.seed,
             "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
             "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],
             "projected_device_seconds": est["seconds"], "projected_device_seconds_se": est["seconds_se"],

@@ -59,7 +59,8 @@ def rmat(scale):
 def pick_shard(graph, pattern, target_ms, first=64, log=None):
     """Roots [lo, lo + size) of the degree-ordered graph, lo = n / 16
     (mid-degree roots), size doubled from `first` until the default config
-    takes target_ms; returns (lo, hi, default seconds)."""
+    takes target_ms or the shard reaches the last root; returns (lo, hi,
+    default seconds)."""
     n = graph.v_cnt()
     lo = n // 16
     size = first
@@ -69,7 +70,7 @@ def pick_shard(graph, pattern, target_ms, first=64, log=None):
         hi = min(n, lo + size)
         plan.count_shard(lo, hi, base)
         _, st = plan.count_shard(lo, hi, base)
-        if st["seconds"] * 1e3 >= target_ms or hi == n or size >= n // 4:
+        if st["seconds"] * 1e3 >= target_ms or hi == n:
             if log:
                 log("  shard [%d, %d): default %.2f ms" % (lo, hi, st["seconds"] * 1e3))
             return lo, hi, st["seconds"]
@@ -98,7 +99,10 @@ def collect(scales, patterns_per_scale, train_pool, per_workload, target_ms, see
         pats = rng.sample(train_pool, patterns_per_scale)
         for p in pats:
             t0 = time.time()
-            lo, hi, _ = pick_shard(g, p, target_ms, log=log)
+            lo, hi, tp = pick_shard(g, p, target_ms, log=log)
+            if tp * 1e3 < target_ms / 4:
+                log("collect s%d %s: %.2f ms on all roots from %d, too small to time; left out" % (sc, p, tp * 1e3, lo))
+                continue
             drv = Driver(g, [p], "unique", reps=1, warmup=1, roots=(lo, hi))
             td = drv.run(default_config())
             feat = featurize(g, p)
@@ -126,19 +130,25 @@ def expected_min_of_random(values, k):
     return float(sum(v[i] * comb(n - 1 - i, k - 1) for i in range(n - k + 1)) / tot)
 
 
-def rank(model, scale, patterns, pool_size, budget, target_ms, seed=0, log=print):
+def rank(model, scale, patterns, pool_size, budget, target_ms, seed=0, log=print, reps=2):
     """Per pattern: measure a seeded pool of configs plus the default (the
-    grid), rank the pool with the model, take its top `budget`; compare with
-    `budget` random picks (exact expectation) and the default config."""
+    grid; median of `reps` runs each), rank the pool with the model, take its
+    top `budget`; compare with `budget` random picks (exact expectation) and
+    the default config. A pattern whose whole graph takes less than
+    target_ms / 4 with the default config is too small to time and is left
+    out (listed in the log)."""
     rng = random.Random(seed + 1)
     sp = space()
     g = rmat(scale)
     out = []
     for p in patterns:
-        lo, hi, _ = pick_shard(g, p, target_ms, log=log)
-        drv = Driver(g, [p], "unique", reps=1, warmup=1, roots=(lo, hi))
-        td = drv.run(default_config())
+        lo, hi, t0 = pick_shard(g, p, target_ms, log=log)
         pool = [default_config()] + sp.random_configs(pool_size, rng)
+        if t0 * 1e3 < target_ms / 4:
+            log("rank s%d %s: %.2f ms on all roots from %d, too small to time; left out" % (scale, p, t0 * 1e3, lo))
+            continue
+        drv = Driver(g, [p], "unique", reps=reps, warmup=1, roots=(lo, hi))
+        td = drv.run(default_config())
         truth = np.array([td] + [drv.run(c) for c in pool[1:]])
         ok = truth < FLOAT_INF
         feat = featurize(g, p)
@@ -184,7 +194,7 @@ def run_study(train_scales, eval_scale, patterns_per_scale, per_workload, pool, 
         "train": {"scales": list(train_scales), "runs": len(y), "workloads": len(set((m["scale"], m["pattern"])
                                                                                   for m in meta)),
                   "loss": loss, "collect_seconds": t_collect},
-        "eval": {"scale": eval_scale, "patterns": len(held), "pool": pool + 1, "budget": budget,
+        "eval": {"scale": eval_scale, "patterns": len(res), "held_out": len(held), "pool": pool + 1, "budget": budget,
                  "seconds": time.time() - t0},
         "mean_model_regret": float(np.mean([r["model_regret"] for r in res])),
         "mean_random_regret": float(np.mean([r["random_regret"] for r in res])),

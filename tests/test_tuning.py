@@ -125,3 +125,25 @@ def test_workload_patterns():
     assert len(b) == 31 == len(set(b)) and b[:21] == connected_patterns(5)
     assert all(len(p) == 36 and 0.4 <= workloads.edge_density(p) <= 0.8 for p in b[21:])
     assert workloads.batched_patterns() == b  # seeded
+
+
+def test_config_study_helpers():
+    """The config-ranking study's no-model baseline is the exact expectation
+    of the best of k random picks (checked against every k-subset), and its
+    space drops the always-linear hub scans and pins the array store."""
+    import itertools
+
+    import numpy as np
+
+    from graphautotune_amd.tuning import config_study as cs
+    rng = np.random.default_rng(5)
+    for n in (1, 4, 7):
+        v = rng.uniform(1, 10, n)
+        for k in range(1, n + 1):
+            brute = np.mean([min(c) for c in itertools.combinations(v, k)])
+            assert cs.expected_min_of_random(v, k) == pytest.approx(brute)
+    assert cs.expected_min_of_random([3.0, 1.0], 5) == pytest.approx(1.0)
+    space = cs.space()
+    names = {p.name: p for p in space.params}
+    assert 64 not in names["search_linear_threshold"].values and "vertex_store" not in names
+    assert all(c.values["vertex_store"] == 1 for c in space.random_configs(20))
