@@ -72,24 +72,24 @@ def strata_of(n, first=1):
     return cuts
 
 
-def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=0, log=None):
+def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=0, log=None):
     """Estimated whole-graph raw sums and device time of `plan` from a
     stratified sample of root blocks, with standard errors.
 
     Roots are ordered by degree (unique and labeled plans run on the
     degree-ordered graph), so the strata of strata_of() hold roots of
-    similar cost. A stratum is cut into at most max_units blocks of
-    consecutive roots; per_stratum of them (seeded, without replacement)
-    are counted with gat_count_shard, or all of them when there are no
-    more. Per stratum h with N_h blocks, n_h sampled, block sums y: total
-    N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) / n_h (exact when
-    n_h = N_h). Device time projects the same way, as a sum of block
-    times; that holds when each timed block has enough roots to fill the GPU
-    (a launch of a few light roots costs its fixed launch time, about a
-    millisecond, whatever its work), hence few, large blocks (max_units of
-    64 or so) for time. Returns {"raw": [...], "raw_se": [...], "seconds",
-    "seconds_se", "strata": [...]} (raw sums are not divided by the
-    divisor)."""
+    similar cost. Each stratum is cut into equal blocks of consecutive
+    roots, the block size grown from one root until a block, drawn at
+    random, takes at least min_block_s (so a timed block fills the GPU and
+    the block times add up to what the whole count takes; each launch stays
+    seconds long). Then per_stratum blocks of that size are drawn without
+    replacement (all blocks when there are no more; the pilot blocks, whose
+    draw decided the size, are not reused) and counted with
+    gat_count_shard. Per stratum h with N_h blocks, n_h sampled,
+    block sums y: total N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) /
+    n_h (exact when n_h = N_h); device time likewise. Returns {"raw": [...],
+    "raw_se": [...], "seconds", "seconds_se", "strata": [...]} (raw sums are
+    not divided by the divisor)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     n = plan.graph.v_cnt()
@@ -97,21 +97,35 @@ def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=
     raw, var = np.zeros(np_), np.zeros(np_)
     sec, sec_var = 0.0, 0.0
     strata = []
+
+    def block(lo, hi, blk, b):
+        r0 = lo + int(b) * blk
+        r1 = min(hi, r0 + blk)
+        got, st = plan.count_shard(r0, r1, config)
+        if log and st["seconds"] > 1.0:
+            log("  block [%d, %d): %.2f s" % (r0, r1, st["seconds"]))
+        return got, st["seconds"]
+
     for lo, hi in strata_of(n):
         size = hi - lo
-        blk = -(-size // max_units)  # ceil: roots per block
+        blk, pilot = 1, 0.0
+        while True:  # pilot: the block size whose launch lasts min_block_s
+            nb = -(-size // blk)
+            b = int(rng.integers(nb))
+            _, t = block(lo, hi, blk, b)
+            pilot += t
+            if t >= min_block_s or blk >= size:
+                break
+            blk = min(size, blk * (2 if t > min_block_s / 4 else 8))
         nb = -(-size // blk)
-        pick = np.arange(nb) if nb <= per_stratum else np.sort(rng.choice(nb, per_stratum, replace=False))
+        pick = range(nb) if nb <= per_stratum else rng.choice(nb, per_stratum, replace=False)
         ys, ts = [], []
-        for b in pick:
-            r0 = lo + int(b) * blk
-            got, st = plan.count_shard(r0, min(hi, r0 + blk), config)
-            ys.append(got)
-            ts.append(st["seconds"])
-            if log and st["seconds"] > 1.0:
-                log("  block [%d, %d): %.2f s" % (r0, min(hi, r0 + blk), st["seconds"]))
+        for x in sorted(int(v) for v in pick):
+            g2, t2 = block(lo, hi, blk, x)
+            ys.append(g2)
+            ts.append(t2)
         ys, ts = np.array(ys, dtype=np.float64), np.array(ts)
-        k = len(pick)
+        k = len(ys)
         tot = nb * ys.mean(axis=0)
         fpc = 1.0 - k / nb
         v = (nb * nb * fpc * ys.var(axis=0, ddof=1) / k) if k > 1 and fpc > 0 else np.zeros(np_)
@@ -123,7 +137,8 @@ def stratified_estimate(plan, config=None, max_units=4096, per_stratum=24, seed=
         sec_var += tv
         strata.append({"roots": [lo, hi], "block": blk, "blocks": nb, "sampled": k,
                        "raw": tot.tolist(), "raw_se": np.sqrt(v).tolist(), "seconds": tsum,
-                       "seconds_se": float(np.sqrt(tv)), "sampled_seconds": float(ts.sum())})
+                       "seconds_se": float(np.sqrt(tv)), "sampled_seconds": float(ts.sum()),
+                       "pilot_seconds": pilot})
         if log:
             log("stratum [%d, %d): %d of %d blocks of %d roots, %.3f s sampled, projected %.2f +- %.2f s"
                 % (lo, hi, k, nb, blk, ts.sum(), tsum, np.sqrt(tv)))
@@ -153,10 +168,9 @@ def main(argv=None):
     ap.add_argument("--estimate", action="store_true",
                     help="project the whole-graph counts and device time from a stratified sample of root "
                          "blocks (see stratified_estimate) instead of counting every root")
-    ap.add_argument("--per-stratum", type=int, default=24, help="with --estimate: blocks sampled per stratum")
-    ap.add_argument("--max-units", type=int, default=4096,
-                    help="with --estimate: blocks a stratum is cut into (fewer, larger blocks make each timed "
-                         "launch long enough to fill the GPU, so their times add up to the whole count's)")
+    ap.add_argument("--per-stratum", type=int, default=8, help="with --estimate: blocks sampled per stratum")
+    ap.add_argument("--min-block", type=float, default=1.0,
+                    help="with --estimate: seconds a sampled block should take (see stratified_estimate)")
     ap.add_argument("--seed", type=int, default=0, help="with --estimate: sampling seed")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="count only roots [LO, HI) and report raw sums (add the raw sums of a partition, "
@@ -173,13 +187,13 @@ def main(argv=None):
     plan = core.Plan(g, pats, args.mode)
     cfg = json.loads(args.config) if args.config else {}
     if args.estimate:
-        est = stratified_estimate(plan, cfg, max_units=args.max_units, per_stratum=args.per_stratum, seed=args.seed,
-                                  log=log)
+        est = stratified_estimate(plan, cfg, per_stratum=args.per_stratum, min_block_s=args.min_block,
+                                  seed=args.seed, log=log)
         infos = [plan.count_info(i) for i in range(len(pats))]
         E = g.undirected_edges()
         print(json.dumps({
             "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
-            "kernel_config": cfg, "per_stratum": args.per_stratum, "max_units": args.max_units, "seed": args.seed,
+            "kernel_config": cfg, "per_stratum": args.per_stratum, "min_block_s": args.min_block, "seed": args.seed,
             "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
             "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],
             "projected_device_seconds": est["seconds"], "projected_device_seconds_se": est["seconds_se"],

@@ -14,6 +14,7 @@ model regresses log(runtime / default runtime) of its workload, which is
 what ranking within a workload needs.
 
     python -m graphautotune_amd.tuning.config_study --out results/config_study_rmat16-20_to_rmat21.json
+    python -m graphautotune_amd.tuning.config_study --judge-clique --out results/clique_grid_rmat20.json
 
 Reference: tuning/tuning/role/modeling.py:81-102,152-162 (fit on measured
 runtimes, predict to rank) and tuning/tuning/tuner.py:28-53 (measure the
@@ -208,6 +209,31 @@ def run_study(train_scales, eval_scale, patterns_per_scale, per_workload, pool, 
     }
 
 
+def judge_clique_config(scale=20, knobs=("queue_chunk", "slot_capacity", "block_waves", "waves_per_cu"), log=print):
+    """The benchmark's tuned K4 + K5 config (tuned_configs.json) against a
+    grid over `knobs` with the other knobs at their tuned values, on the
+    benchmark graph: the grid is the judge of the tuner's pick."""
+    import os
+    from .space import BY_NAME
+    from .tuner import grid_search
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_configs.json")
+    with open(path) as f:
+        tuned = json.load(f)["rmat%d-k4k5" % scale]["config"]
+    g = core.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, scale)
+    pats = [workloads.K4, workloads.K5]
+    drv = Driver(g, pats, "unique", reps=3, warmup=1)
+    t_tuned = drv.run(Config(tuned))
+    fixed = {k: v for k, v in tuned.items() if k not in knobs}
+    best, t_best, results = grid_search(drv, ConfigSpace(list(knobs), fixed=fixed), log=log)
+    ranks = sorted(t for _, t in results)
+    return {"graph": "rmat%d" % scale, "patterns": pats, "knobs": list(knobs),
+            "grid_points": len(results), "grid_values": {k: BY_NAME[k].values for k in knobs},
+            "tuned_config": tuned, "tuned_ms": t_tuned * 1e3, "grid_best_config": best.as_dict(),
+            "grid_best_ms": t_best * 1e3, "tuned_regret": t_tuned / t_best - 1.0,
+            "tuned_rank": sum(1 for t in ranks if t < t_tuned) + 1, "counts": drv.counts,
+            "grid": [[c.as_dict(), t * 1e3] for c, t in results]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--train-scales", default="16,17,18,19,20")
@@ -220,8 +246,19 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=800)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--judge-clique", action="store_true",
+                    help="instead: time the benchmark's tuned K4 + K5 config against a grid of queue chunk x "
+                         "slot capacity x block waves x waves per CU on R-MAT scale 20")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    if args.judge_clique:
+        res = judge_clique_config(log=log)
+        res["command"] = "python -m graphautotune_amd.tuning.config_study " + " ".join(argv or sys.argv[1:])
+        print(json.dumps({k: v for k, v in res.items() if k != "grid"}), flush=True)
+        if args.out:
+            with open(args.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return
     summary = run_study([int(s) for s in args.train_scales.split(",")], args.eval_scale, args.patterns_per_scale,
                         args.per_workload, args.pool, args.budget, args.target_ms, args.steps, args.seed, log)
     summary["command"] = "python -m graphautotune_amd.tuning.config_study " + " ".join(argv or sys.argv[1:])
