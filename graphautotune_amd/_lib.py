@@ -182,6 +182,8 @@ _SIGS = {
     "gat_plan_count_info": (C.c_int, [_P, C.c_int32, C.POINTER(CountInfo), C.c_size_t]),
     "gat_count_shard": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_uint32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
+    "gat_count_shard_edges": (C.c_int, [_P, C.POINTER(KernelConfig), C.c_uint32, C.c_int64, C.c_int64,
+                                        C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "gat_count_cliques_noalloc": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.POINTER(KernelConfig), _P,
                                             C.POINTER(C.c_uint64)]),
     "gat_plan_free": (None, [_P]),

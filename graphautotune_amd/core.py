@@ -351,6 +351,33 @@ class Plan:
         return [int(x) for x in out], stats
 
 
+    def count_shard_edges(self, edge_begin, edge_end, config=None, account=False):
+        """Raw sums (one per pattern) of the embeddings whose first two
+        Schedule vertices map to a directed edge with index in [edge_begin,
+        edge_end) of the CSR the plan runs on (array store only), and a
+        stats dict. Over a partition of the edges the raw sums add up to
+        divisor * count."""
+        n = len(self.patterns)
+        out = (C.c_uint64 * n)()
+        st = CountStats()
+        cfg = config if isinstance(config, KernelConfig) else KernelConfig.from_dict(config or {})
+        flags = _lib.FLAG_ACCOUNT if account else 0
+        check(lib().gat_count_shard_edges(self._h, C.byref(cfg), flags, int(edge_begin), int(edge_end), out,
+                                          C.byref(st)))
+        return [int(x) for x in out], _stats_dict(st)
+
+    def run_offsets(self):
+        """int64 offsets of the CSR the plan runs on: the input's for
+        reference-mode plans, else the degree-ordered graph's (prefix sums of
+        the degrees sorted in descending order)."""
+        import numpy as np
+        off, _ = self.graph.csr()
+        deg = np.diff(np.asarray(off, dtype=np.int64))
+        if self.count_info(0)["relabeled"]:
+            deg = np.sort(deg)[::-1]
+        return np.concatenate([[0], np.cumsum(deg)]).astype(np.int64)
+
+
 def count(graph, patterns, mode="unique", config=None, account=False):
     """Exact counts of every pattern in one batched launch."""
     plan = Plan(graph, patterns, mode)

@@ -2418,6 +2418,29 @@ extern "C" int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32
     return finish(p, L, raw_out, stats, true, true);
 }
 
+extern "C" int gat_count_shard_edges(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags, int64_t edge_begin,
+                                     int64_t edge_end, uint64_t *raw_out, gat_count_stats *stats) {
+    if (!p) GAT_FAIL(GAT_EINVAL, "null plan");
+    const long long m = p->graph->m;
+    if (edge_begin < 0 || edge_end < edge_begin || edge_end > m)
+        GAT_FAIL(GAT_EINVAL, "edge range [%lld, %lld) is not within [0, %lld)", (long long)edge_begin,
+                 (long long)edge_end, m);
+    Launch L;
+    int rc = prepare(p, cfg, &L);
+    if (rc) return rc;
+    if (L.clique)
+        GAT_FAIL(GAT_EUNSUPPORTED, "edge shards need the array store (vertex_store = 1): the bitmap clique store "
+                                   "shards by top vertex (gat_count_shard)");
+    // the array store's queue runs over root edges; every task finds its
+    // root by binary search, so any edge range is a shard
+    L.a.e_lo = L.ao.e_lo = edge_begin;
+    L.a.e_hi = L.ao.e_hi = edge_end;
+    L.tasks = edge_end - edge_begin;
+    rc = enqueue(p->dev, L, (flags & GAT_FLAG_ACCOUNT) != 0, true, (flags & GAT_FLAG_TIMELINE) != 0);
+    if (rc) return rc;
+    return finish(p, L, raw_out, stats, true, true);
+}
+
 // ---- allocation-free clique count ---------------------------------------
 // Scratch of gat_count_cliques_noalloc, one copy per device for the life of
 // the process: kNaHead words of counters (counts [0,8), queues [8,24),

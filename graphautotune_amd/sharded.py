@@ -22,22 +22,51 @@ from . import core, workloads
 PATTERNS = workloads.NAMED
 
 
-def raw_sharded(plan, config=None, target_s=5.0, first=1, roots=None, log=None):
+def edge_shards(plan, config=None):
+    """Whether the plan's count can be split by directed edges
+    (gat_count_shard_edges: the array store). Clique plans on the bitmap
+    store are split by top vertex."""
+    from ._lib import GAT_EUNSUPPORTED, GatError
+    try:
+        plan.count_shard_edges(0, 0, config)
+        return True
+    except GatError as e:
+        if e.code == GAT_EUNSUPPORTED:
+            return False
+        raise
+
+
+def raw_sharded(plan, config=None, target_s=5.0, first=1, roots=None, log=None, quiet_below=0.0):
     """Raw sums of every pattern of `plan` over the roots [lo, hi) (default:
-    all of them) and a list of (lo, hi, seconds) shards."""
-    lo, n = roots if roots else (0, plan.graph.v_cnt())
+    all of them) and a list of (lo, hi, seconds) shards. Launches are sized
+    to about target_s each: by directed edges of the CSR the plan runs on for
+    the array store (so one heavy root's work spreads over launches), by
+    roots for the bitmap clique store; the first shard has `first` units and
+    each next one grows or shrinks toward target_s (at most 4x). Shard
+    bounds are edge indices or root ids accordingly. Shards shorter than
+    quiet_below seconds are not logged."""
+    r_lo, r_hi = roots if roots else (0, plan.graph.v_cnt())
+    if edge_shards(plan, config):
+        off = plan.run_offsets()
+        lo, n = int(off[r_lo]), int(off[r_hi])
+        size, unit = max(1, first), "edges"
+        count = lambda a, b: plan.count_shard_edges(a, b, config)  # noqa: E731
+    else:
+        lo, n = r_lo, r_hi
+        size, unit = max(1, first), "roots"
+        count = lambda a, b: plan.count_shard(a, b, config)  # noqa: E731
+    start = lo
     np_ = len(plan.patterns)
     raw = [0] * np_
     shards = []
-    size = max(1, first)
     while lo < n:
         hi = min(n, lo + size)
-        got, st = plan.count_shard(lo, hi, config)
+        got, st = count(lo, hi)
         raw = [a + b for a, b in zip(raw, got)]
         shards.append((lo, hi, st["seconds"]))
-        if log:
-            log("shard [%d, %d): %.3f s, split tasks %d; %.1f %% of roots done"
-                % (lo, hi, st["seconds"], st["split_tasks"], 100.0 * hi / n))
+        if log and st["seconds"] >= quiet_below:
+            log("shard [%d, %d) %s: %.3f s, split tasks %d; %.1f %% done"
+                % (lo, hi, unit, st["seconds"], st["split_tasks"], 100.0 * (hi - start) / max(1, n - start)))
         t = max(st["seconds"], 1e-4)
         size = max(1, int(size * min(4.0, max(0.25, target_s / t))))
         lo = hi
@@ -74,25 +103,34 @@ def strata_of(n, first=1):
 
 def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=0, log=None):
     """Estimated whole-graph raw sums and device time of `plan` from a
-    stratified sample of root blocks, with standard errors.
+    stratified sample of blocks, with standard errors.
 
-    Roots are ordered by degree (unique and labeled plans run on the
-    degree-ordered graph), so the strata of strata_of() hold roots of
-    similar cost. Each stratum is cut into equal blocks of consecutive
-    roots, the block size grown from one root until a block, drawn at
-    random, takes at least min_block_s (so a timed block fills the GPU and
-    the block times add up to what the whole count takes; each launch stays
-    seconds long). Then per_stratum blocks of that size are drawn without
-    replacement (all blocks when there are no more; the pilot blocks, whose
-    draw decided the size, are not reused) and counted with
-    gat_count_shard. Per stratum h with N_h blocks, n_h sampled,
-    block sums y: total N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) /
-    n_h (exact when n_h = N_h); device time likewise. Returns {"raw": [...],
-    "raw_se": [...], "seconds", "seconds_se", "strata": [...]} (raw sums are
-    not divided by the divisor)."""
+    The sampling units are the directed edges of the CSR the plan runs on
+    for the array store (gat_count_shard_edges) and the roots for the bitmap
+    clique store (gat_count_shard). Both are in degree order (unique and
+    labeled plans run on the degree-ordered graph; an edge's index follows
+    its root's degree, then its neighbour's), so the strata of strata_of()
+    over the unit indices hold units of similar cost; units of a hub root
+    fall into strata of their own. Each stratum is cut into equal blocks of
+    consecutive units, the block size grown from one unit until a block,
+    drawn at random, takes at least min_block_s (so a timed block fills the
+    GPU and each launch stays seconds long). Then per_stratum blocks of that
+    size are drawn without replacement (all blocks when there are no more;
+    the pilot blocks, whose draw decided the size, are not reused) and
+    counted. Per stratum h with N_h blocks, n_h sampled, block sums y: total
+    N_h * mean(y), variance N_h^2 (1 - n_h/N_h) var(y) / n_h (exact when
+    n_h = N_h); device time likewise (an upper bound where blocks are a few
+    units: a launch of few tasks runs its longest one alone). Returns
+    {"raw": [...], "raw_se": [...], "seconds", "seconds_se", "unit",
+    "strata": [...]} (raw sums are not divided by the divisor)."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    n = plan.graph.v_cnt()
+    if edge_shards(plan, config):
+        n, unit = int(plan.run_offsets()[-1]), "edges"
+        count = lambda a, b: plan.count_shard_edges(a, b, config)  # noqa: E731
+    else:
+        n, unit = plan.graph.v_cnt(), "roots"
+        count = lambda a, b: plan.count_shard(a, b, config)  # noqa: E731
     np_ = len(plan.patterns)
     raw, var = np.zeros(np_), np.zeros(np_)
     sec, sec_var = 0.0, 0.0
@@ -101,9 +139,9 @@ def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=
     def block(lo, hi, blk, b):
         r0 = lo + int(b) * blk
         r1 = min(hi, r0 + blk)
-        got, st = plan.count_shard(r0, r1, config)
+        got, st = count(r0, r1)
         if log and st["seconds"] > 1.0:
-            log("  block [%d, %d): %.2f s" % (r0, r1, st["seconds"]))
+            log("  block [%d, %d) %s: %.2f s" % (r0, r1, unit, st["seconds"]))
         return got, st["seconds"]
 
     for lo, hi in strata_of(n):
@@ -135,15 +173,15 @@ def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=
         var += v
         sec += tsum
         sec_var += tv
-        strata.append({"roots": [lo, hi], "block": blk, "blocks": nb, "sampled": k,
+        strata.append({"units": [lo, hi], "block": blk, "blocks": nb, "sampled": k,
                        "raw": tot.tolist(), "raw_se": np.sqrt(v).tolist(), "seconds": tsum,
                        "seconds_se": float(np.sqrt(tv)), "sampled_seconds": float(ts.sum()),
                        "pilot_seconds": pilot})
         if log:
-            log("stratum [%d, %d): %d of %d blocks of %d roots, %.3f s sampled, projected %.2f +- %.2f s"
-                % (lo, hi, k, nb, blk, ts.sum(), tsum, np.sqrt(tv)))
+            log("stratum [%d, %d) %s: %d of %d blocks of %d, %.3f s sampled, projected %.2f +- %.2f s"
+                % (lo, hi, unit, k, nb, blk, ts.sum(), tsum, np.sqrt(tv)))
     return {"raw": raw.tolist(), "raw_se": np.sqrt(var).tolist(), "seconds": sec,
-            "seconds_se": float(np.sqrt(sec_var)), "strata": strata}
+            "seconds_se": float(np.sqrt(sec_var)), "unit": unit, "strata": strata}
 
 
 def make_graph(spec):

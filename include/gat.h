@@ -321,6 +321,18 @@ int gat_plan_count_info(const gat_plan *p, int32_t i, gat_pattern_count_info *ou
 int gat_count_shard(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
                     int32_t root_begin, int32_t root_end, uint64_t *raw_out,
                     gat_count_stats *stats);
+/* A finer shard for the array store: only the embeddings whose first two
+ * Schedule vertices map to a directed edge (root, neighbour) with index in
+ * [edge_begin, edge_end) of the CSR the plan runs on (for relabelled plans
+ * the degree-ordered CSR: offsets are the prefix sums of the degrees sorted
+ * in descending order). Raw sums of a partition of [0, e_directed) add up to
+ * divisor * count, like those of gat_count_shard; one heavy root's work can
+ * so be split across launches or GPUs. Clique plans on the bitmap store
+ * fail with GAT_EUNSUPPORTED (they shard by top vertex). No counterpart in
+ * the reference (engine.cuh:246-261 splits the first vertex's range only). */
+int gat_count_shard_edges(gat_plan *p, const gat_kernel_config *cfg, uint32_t flags,
+                          int64_t edge_begin, int64_t edge_end, uint64_t *raw_out,
+                          gat_count_stats *stats);
 void gat_plan_free(gat_plan *p);
 
 /* Allocation-free clique count: #K3 .. #K_kmax (kmax in [3, 6]) of a graph

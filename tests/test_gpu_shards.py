@@ -160,7 +160,15 @@ def test_chung_lu_full_house_c5_properties(gat, gpu, chung_lu_full):
     assert house.count_shard(mid, mid + 256, {"hub_degree": -1})[0][0] == part
 
 
-def test_rmat22_batched_31_shards(gat, gpu):
+def test_rmat22_batched_31_shards(gat, orc, gpu):
+    """The batched configuration's graph (R-MAT scale 22) and 31 patterns:
+    batched equals plain and shards add up on mid-degree roots and on the
+    64 lightest roots of degree >= 2; on the 64 largest hubs the oracle's
+    restricted count is reached for the patterns whose restrictions make the
+    root the largest id of every embedding (the rest of the embedding then
+    lies among the hubs; for the other patterns a root's embeddings run
+    through every hub neighbourhood, beyond the oracle at this size)."""
+    import numpy as np
     from graphautotune_amd import workloads
     g = workloads.graph("rmat22")
     pats = workloads.batched_patterns()
@@ -173,6 +181,26 @@ def test_rmat22_batched_31_shards(gat, gpu):
     assert [x + y for x, y in zip(*halves)] == b
     say("rmat22 plain shards")
     assert b == [gat.Plan(g, [p], "unique").count_shard(lo, lo + 4)[0][0] for p in pats]
+    off, adj = g.csr()
+    roff, radj = orc.relabel(off, adj)
+    deg = np.diff(roff)
+    hi = int(np.nonzero(deg >= 2)[0][-1]) + 1
+    light = (hi - 64, hi)
+    say("rmat22 light roots", light)
+    lb = batched.count_shard(*light)[0]
+    assert lb == [gat.Plan(g, [p], "unique").count_shard(*light)[0][0] for p in pats]
+    assert [x + y for x, y in zip(batched.count_shard(light[0], light[0] + 20)[0],
+                                  batched.count_shard(light[0] + 20, light[1])[0])] == lb
+    checked = 0
+    for i in range(len(pats)):
+        plan = gat.Plan(g, [pats[i]], "unique")
+        rows, info = ordered(plan, 0)
+        if {b_ for a_, b_ in info["restrictions"] if a_ == 0} == set(range(1, len(rows))):
+            say("rmat22 hub core oracle", pats[i])
+            assert plan.count_shard(0, 64)[0][0] == orc.restricted_count(roff, radj, rows, info["restrictions"],
+                                                                         0, 64), pats[i]
+            checked += 1
+    assert checked >= 2  # K5 and the 5-vertex pattern with restrictions (0,1) .. (0,4)
 
 
 def test_count_sharded_equals_count(gat, gpu):
@@ -206,3 +234,36 @@ def test_reference_mode_shards_match_oracle(gat, orc, gpu):
             for cfg in ({}, OVERFLOW):
                 got = [plan.count_shard(lo, hi, cfg)[0][0] for lo, hi in zip(cuts, cuts[1:])]
                 assert got == [v for v, _ in want], (p, cfg)
+
+
+def test_edge_shards_partition_counts(gat, gpu):
+    """gat_count_shard_edges: raw sums over a partition of the directed edges
+    of the plan's CSR add up to the root shards' and to divisor x count, in
+    unique and reference modes (reference plans run on the input ids), with
+    splitting and overflow; a heavy root's edges split across shards; clique
+    plans on the bitmap store refuse edge shards, on the array store take
+    them."""
+    from graphautotune_amd._lib import GAT_EUNSUPPORTED, GatError
+    g = gat.Graph.chung_lu(400, 10.0, 2.1, 80, 3)
+    p6 = connected_patterns(6)[40]
+    for pats, mode in (([HOUSE, C5, p6], "unique"), ([HOUSE, K4], "reference")):
+        plan = gat.Plan(g, pats, mode)
+        off = plan.run_offsets()
+        m = int(off[-1])
+        whole = plan.count_shard(0, g.v_cnt())[0]
+        for cfg in ({}, OVERFLOW, {"split_us": 1, "vertex_store": 1}):
+            cuts = [0, 1, int(off[1]) // 2, int(off[1]), m // 3, m - 1, m]
+            parts = [plan.count_shard_edges(a, b, cfg)[0] for a, b in zip(cuts, cuts[1:])]
+            assert [sum(x) for x in zip(*parts)] == whole, (pats, mode, cfg)
+        if mode == "unique":
+            counts = plan.count()[0]
+            infos = [plan.count_info(i) for i in range(len(pats))]
+            assert [w // i["divisor"] * i["multiplier"] for w, i in zip(whole, infos)] == counts
+    clique = gat.Plan(g, [K4], "unique")
+    with pytest.raises(GatError) as e:
+        clique.count_shard_edges(0, 10)
+    assert e.value.code == GAT_EUNSUPPORTED
+    m = int(clique.run_offsets()[-1])
+    halves = [clique.count_shard_edges(0, m // 2, {"vertex_store": 1})[0][0],
+              clique.count_shard_edges(m // 2, m, {"vertex_store": 1})[0][0]]
+    assert sum(halves) == clique.count_shard(0, g.v_cnt())[0][0]

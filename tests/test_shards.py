@@ -86,6 +86,10 @@ class StubPlan:
         work = sum(self.per_root[lo:hi])
         return [work], {"seconds": 1e-3 * (hi - lo) + 1e-6 * work, "split_tasks": 0}
 
+    def count_shard_edges(self, lo, hi, config=None):  # a bitmap-store (clique) plan: root shards only
+        from graphautotune_amd._lib import GAT_EUNSUPPORTED, GatError
+        raise GatError(GAT_EUNSUPPORTED, "edge shards need the array store")
+
     def count_info(self, i):
         return {"divisor": self.divisor, "multiplier": self.multiplier, "relabeled": True, "restrictions": []}
 
@@ -121,7 +125,11 @@ class _StubPlan:
 
     def count_shard(self, lo, hi, cfg=None):
         self.calls += 1
-        return [int(self.values[lo:hi].sum())], {"seconds": 1e-3 + float(self.costs[lo:hi].sum())}
+        return [int(self.values[lo:hi].sum())], {"seconds": 1e-3 + float(self.costs[lo:hi].sum()), "split_tasks": 0}
+
+    def count_shard_edges(self, lo, hi, config=None):
+        from graphautotune_amd._lib import GAT_EUNSUPPORTED, GatError
+        raise GatError(GAT_EUNSUPPORTED, "edge shards need the array store")
 
 
 def test_stratified_estimate_exact_and_unbiased():
@@ -149,3 +157,46 @@ def test_stratified_estimate_exact_and_unbiased():
     assert abs(np.mean(ests) - values.sum()) < 3 * np.mean(ses) / np.sqrt(len(ests)) + 1e-9 * values.sum()
     last = e["strata"][-1]
     assert last["block"] > 1 and last["sampled"] == 4
+
+
+class _EdgeStubPlan:
+    """An array-store plan over a CSR of known per-edge raw values: edge
+    shards sum them; root shards are edge shards over the roots' edges."""
+
+    def __init__(self, degrees, seed=1):
+        import numpy as np
+        rng = np.random.default_rng(seed)
+        self.off = np.concatenate([[0], np.cumsum(degrees)]).astype(np.int64)
+        self.per_edge = rng.integers(0, 1000, int(self.off[-1]))
+        self.patterns = ["p"]
+        n = len(degrees)
+        self.graph = type("G", (), {"v_cnt": lambda _self: n})()
+        self.edge_calls = []
+
+    def run_offsets(self):
+        return self.off
+
+    def count_shard_edges(self, lo, hi, config=None):
+        self.edge_calls.append((lo, hi))
+        return [int(self.per_edge[lo:hi].sum())], {"seconds": 1e-3 + 1e-6 * (hi - lo), "split_tasks": 0}
+
+    def count_shard(self, lo, hi, config=None):
+        return self.count_shard_edges(int(self.off[lo]), int(self.off[hi]), config)
+
+
+def test_edge_shards_partition_a_heavy_root():
+    """Array-store plans are walked in directed-edge shards: the shards
+    partition the requested roots' edges (one heavy root's edges fall into
+    several launches) and the raw sums add up."""
+    import numpy as np
+
+    from graphautotune_amd.sharded import raw_sharded
+    degrees = np.array([200000, 5, 0, 7, 3] + [1] * 100)
+    plan = _EdgeStubPlan(degrees)
+    raw, shards = raw_sharded(plan, target_s=0.02, first=1)
+    assert raw == [int(plan.per_edge.sum())]
+    assert [s[0] for s in shards[1:]] == [s[1] for s in shards[:-1]]
+    assert shards[0][0] == 0 and shards[-1][1] == int(plan.off[-1])
+    assert sum(1 for s in shards if s[1] <= 200000) > 3  # root 0 alone spans several launches
+    raw2, shards2 = raw_sharded(plan, target_s=0.02, first=1, roots=(3, 50))
+    assert raw2 == [int(plan.per_edge[plan.off[3]:plan.off[50]].sum())]
