@@ -101,7 +101,46 @@ def strata_of(n, first=1):
     return cuts
 
 
-def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=0, log=None):
+def unit_space(plan, config=None):
+    """(units, unit name, count function, first unit of each root) of the
+    plan's sampling units: directed edges of the CSR the plan runs on for
+    the array store, roots for the bitmap clique store."""
+    if edge_shards(plan, config):
+        off = plan.run_offsets()
+        return (int(off[-1]), "edges", lambda a, b: plan.count_shard_edges(a, b, config),
+                lambda r: int(off[r]))
+    return plan.graph.v_cnt(), "roots", lambda a, b: plan.count_shard(a, b, config), int
+
+
+def head_ladder(plan, config=None, limit_s=20.0, start_root=65536, log=None):
+    """Times single units at the heavy end of the degree order: the first
+    unit of roots start_root, start_root/2, ..., 1, 0 (for the array store
+    the edge from that root to its highest-degree neighbour, the root's most
+    expensive edge). Stops after the first unit that takes limit_s or more,
+    or at root 0. A unit is one launch and cannot be split further, so a
+    launch of the units before the last root under the limit may run for
+    minutes. Returns ([{"root", "unit", "seconds"}...], the smallest probed
+    root whose unit stayed under limit_s; 0 when all did)."""
+    n, unit, count, first = unit_space(plan, config)
+    r = min(start_root, plan.graph.v_cnt() - 1)
+    probes, ok = [], r + 1
+    while True:
+        u = first(r)
+        if u < n:
+            _, st = count(u, u + 1)
+            probes.append({"root": r, "unit": u, "seconds": st["seconds"]})
+            if log:
+                log("head probe: root %d, %s [%d, %d): %.3f s" % (r, unit, u, u + 1, st["seconds"]))
+            if st["seconds"] >= limit_s:
+                break
+            ok = r
+        if r == 0:
+            break
+        r //= 2
+    return probes, ok
+
+
+def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=0, log=None, units=None):
     """Estimated whole-graph raw sums and device time of `plan` from a
     stratified sample of blocks, with standard errors.
 
@@ -122,15 +161,13 @@ def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=
     n_h = N_h); device time likewise (an upper bound where blocks are a few
     units: a launch of few tasks runs its longest one alone). Returns
     {"raw": [...], "raw_se": [...], "seconds", "seconds_se", "unit",
-    "strata": [...]} (raw sums are not divided by the divisor)."""
+    "strata": [...]} (raw sums are not divided by the divisor). With
+    units=(lo, hi) only the units [lo, hi) are sampled and the sums are
+    theirs alone."""
     import numpy as np
     rng = np.random.default_rng(seed)
-    if edge_shards(plan, config):
-        n, unit = int(plan.run_offsets()[-1]), "edges"
-        count = lambda a, b: plan.count_shard_edges(a, b, config)  # noqa: E731
-    else:
-        n, unit = plan.graph.v_cnt(), "roots"
-        count = lambda a, b: plan.count_shard(a, b, config)  # noqa: E731
+    n, unit, count, _ = unit_space(plan, config)
+    u_lo, u_hi = units if units else (0, n)
     np_ = len(plan.patterns)
     raw, var = np.zeros(np_), np.zeros(np_)
     sec, sec_var = 0.0, 0.0
@@ -144,7 +181,8 @@ def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=
             log("  block [%d, %d) %s: %.2f s" % (r0, r1, unit, st["seconds"]))
         return got, st["seconds"]
 
-    for lo, hi in strata_of(n):
+    for lo, hi in strata_of(u_hi - u_lo):
+        lo, hi = lo + u_lo, hi + u_lo
         size = hi - lo
         blk, pilot = 1, 0.0
         while True:  # pilot: the block size whose launch lasts min_block_s
@@ -210,6 +248,10 @@ def main(argv=None):
     ap.add_argument("--min-block", type=float, default=1.0,
                     help="with --estimate: seconds a sampled block should take (see stratified_estimate)")
     ap.add_argument("--seed", type=int, default=0, help="with --estimate: sampling seed")
+    ap.add_argument("--head-limit", type=float, default=None, metavar="S",
+                    help="with --estimate: first time single units of roots 65536, 32768, ..., 0 (head_ladder), "
+                         "stop at the first that takes S seconds or more, and sample only the units from the "
+                         "last root under S on; the roots before it are reported as excluded")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="count only roots [LO, HI) and report raw sums (add the raw sums of a partition, "
                          "then divide by the divisor and multiply by the multiplier)")
@@ -225,17 +267,24 @@ def main(argv=None):
     plan = core.Plan(g, pats, args.mode)
     cfg = json.loads(args.config) if args.config else {}
     if args.estimate:
+        head, probes = 0, []
+        n_units, _, _, first_unit = unit_space(plan, cfg)
+        if args.head_limit is not None:
+            probes, head = head_ladder(plan, cfg, args.head_limit, log=log)
         est = stratified_estimate(plan, cfg, per_stratum=args.per_stratum, min_block_s=args.min_block,
-                                  seed=args.seed, log=log)
+                                  seed=args.seed, log=log, units=(first_unit(head), n_units))
         infos = [plan.count_info(i) for i in range(len(pats))]
         E = g.undirected_edges()
         print(json.dumps({
             "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
             "kernel_config": cfg, "per_stratum": args.per_stratum, "min_block_s": args.min_block, "seed": args.seed,
+            "head_limit_s": args.head_limit, "head_probes": probes, "excluded_roots": [0, head],
+            "excluded_units": [0, first_unit(head)], "units": n_units, "unit": est["unit"],
             "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
             "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],
             "projected_device_seconds": est["seconds"], "projected_device_seconds_se": est["seconds_se"],
-            "projected_edges_per_s": E / est["seconds"] if est["seconds"] > 0 else None,
+            "projected_edges_per_s": E / est["seconds"] if est["seconds"] > 0 and head == 0 else None,
+            "projected_units_per_s": (n_units - first_unit(head)) / est["seconds"] if est["seconds"] > 0 else None,
             "sampled_device_seconds": sum(st["sampled_seconds"] for st in est["strata"]),
             "strata": est["strata"]}), flush=True)
         return

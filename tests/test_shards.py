@@ -200,3 +200,34 @@ def test_edge_shards_partition_a_heavy_root():
     assert sum(1 for s in shards if s[1] <= 200000) > 3  # root 0 alone spans several launches
     raw2, shards2 = raw_sharded(plan, target_s=0.02, first=1, roots=(3, 50))
     assert raw2 == [int(plan.per_edge[plan.off[3]:plan.off[50]].sum())]
+
+
+def test_head_ladder_and_unit_range_estimate():
+    """head_ladder times the first edge of roots 65536, ..., 1, 0 and stops
+    at the first that reaches the limit; an estimate over a unit range sums
+    those units only (exactly, when every block is sampled)."""
+    import numpy as np
+
+    from graphautotune_amd.sharded import head_ladder, stratified_estimate
+
+    class Timed(_EdgeStubPlan):
+        def count_shard_edges(self, lo, hi, config=None):
+            got, st = super().count_shard_edges(lo, hi, config)
+            # an edge of root r costs 100 / (1 + r) seconds
+            roots = np.searchsorted(self.off, np.arange(lo, hi), side="right") - 1
+            st["seconds"] = float((100.0 / (1 + roots)).sum())
+            return got, st
+
+    degrees = np.array([3] * 70000)
+    plan = Timed(degrees)
+    probes, head = head_ladder(plan, limit_s=10.0)
+    assert [p["root"] for p in probes] == [65536, 32768, 16384, 8192, 4096, 2048, 1024, 512, 256, 128,
+                                           64, 32, 16, 8]
+    assert all(p["unit"] == 3 * p["root"] for p in probes)
+    assert probes[-1]["seconds"] >= 10.0 and head == 16
+    lo = int(plan.off[head])
+    est = stratified_estimate(plan, per_stratum=10 ** 9, min_block_s=0.0, units=(lo, int(plan.off[-1])))
+    assert est["raw"][0] == pytest.approx(float(plan.per_edge[lo:].sum())) and est["raw_se"][0] == 0.0
+    assert est["strata"][0]["units"][0] == lo and est["strata"][-1]["units"][1] == int(plan.off[-1])
+    sampled = [c for c in plan.edge_calls[len(probes) + 1:] if c[1] > c[0]]  # not the empty edge_shards checks
+    assert sampled and min(c[0] for c in sampled) >= lo
