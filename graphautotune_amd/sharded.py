@@ -240,6 +240,9 @@ def main(argv=None):
                     help="add the batched configuration's 31 patterns (workloads.batched_patterns)")
     ap.add_argument("--mode", default="unique", choices=["unique", "labeled", "reference"])
     ap.add_argument("--config", default=None, help="JSON kernel config")
+    ap.add_argument("--schedule-ranks", default=None, metavar="JSON",
+                    help="a result of tuning.batched_study: run its per-pattern Schedule ranks (ranked "
+                         "policy); its pattern list must equal the patterns given")
     ap.add_argument("--target", type=float, default=5.0, help="seconds per shard")
     ap.add_argument("--estimate", action="store_true",
                     help="project the whole-graph counts and device time from a stratified sample of root "
@@ -264,7 +267,14 @@ def main(argv=None):
     pats = [PATTERNS.get(p, p) for p in args.pattern] + (workloads.batched_patterns() if args.batched_set else [])
     if not pats:
         ap.error("no patterns: give --pattern or --batched-set")
-    plan = core.Plan(g, pats, args.mode)
+    sched = {}
+    if args.schedule_ranks:
+        with open(args.schedule_ranks) as f:
+            tuned = json.load(f)
+        if [core.Pattern(p).adj for p in tuned["patterns"]] != [core.Pattern(p).adj for p in pats]:
+            ap.error("%s holds Schedule ranks for other patterns" % args.schedule_ranks)
+        sched = {"schedule": "ranked", "schedule_rank": list(tuned["ranks"])}
+    plan = core.Plan(g, pats, args.mode, **sched)
     cfg = json.loads(args.config) if args.config else {}
     if args.estimate:
         head, probes = 0, []
@@ -277,8 +287,8 @@ def main(argv=None):
         E = g.undirected_edges()
         print(json.dumps({
             "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
-            "kernel_config": cfg, "per_stratum": args.per_stratum, "min_block_s": args.min_block, "seed": args.seed,
-            "head_limit_s": args.head_limit, "head_probes": probes, "excluded_roots": [0, head],
+            "kernel_config": cfg, "schedule_ranks": sched.get("schedule_rank"), "per_stratum": args.per_stratum,
+            "min_block_s": args.min_block, "seed": args.seed, "head_limit_s": args.head_limit, "head_probes": probes, "excluded_roots": [0, head],
             "excluded_units": [0, first_unit(head)], "units": n_units, "unit": est["unit"],
             "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
             "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],

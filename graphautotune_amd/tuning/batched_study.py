@@ -57,7 +57,7 @@ def main(argv=None):
         if c[0] != cb[i]:
             raise AssertionError("pattern %d: plain %d vs batched %d" % (i, c[0], cb[i]))
         plain.append(t)
-    out = {"patterns": len(pats), "proxy": "R-MAT s%d" % args.proxy_scale, "graph": "R-MAT s%d" % args.scale,
+    out = {"patterns": pats, "proxy": "R-MAT s%d" % args.proxy_scale, "graph": "R-MAT s%d" % args.scale,
            "vertices": g.v_cnt(), "undirected_edges": g.undirected_edges(), "ranks": ranks,
            "tuning_seconds": tune_s, "batched_tuned_ms": tb * 1e3, "batched_reference_ms": tr * 1e3,
            "plain_reference_ms": sum(plain) * 1e3, "batched_tuned_vs_plain": sum(plain) / tb, "counts": cb}

@@ -60,22 +60,29 @@ def rmat(scale):
 def pick_shard(graph, pattern, target_ms, first=64, log=None):
     """Roots [lo, lo + size) of the degree-ordered graph, lo = n / 16
     (mid-degree roots), size doubled from `first` until the default config
-    takes target_ms or the shard reaches the last root; returns (lo, hi,
+    takes target_ms or the shard reaches the last root, or halved while it
+    takes more than twice target_ms (down to one root); returns (lo, hi,
     default seconds)."""
     n = graph.v_cnt()
     lo = n // 16
     size = first
     plan = core.Plan(graph, [pattern], "unique")
     base = dict(FIXED)
+    grown = False
     while True:
         hi = min(n, lo + size)
         plan.count_shard(lo, hi, base)
         _, st = plan.count_shard(lo, hi, base)
-        if st["seconds"] * 1e3 >= target_ms or hi == n:
+        ms = st["seconds"] * 1e3
+        if ms > 2 * target_ms and size > 1 and not grown:
+            size //= 2
+            continue
+        if ms >= target_ms or hi == n or (size == 1 and not grown) or ms > 2 * target_ms:
             if log:
-                log("  shard [%d, %d): default %.2f ms" % (lo, hi, st["seconds"] * 1e3))
+                log("  shard [%d, %d): default %.2f ms" % (lo, hi, ms))
             return lo, hi, st["seconds"]
-        size *= 2 if st["seconds"] * 1e3 > target_ms / 8 else 8
+        grown = True
+        size *= 2 if ms > target_ms / 8 else 8
 
 
 def featurize(graph, pattern):
@@ -107,8 +114,10 @@ def collect(scales, patterns_per_scale, train_pool, per_workload, target_ms, see
             drv = Driver(g, [p], "unique", reps=1, warmup=1, roots=(lo, hi))
             td = drv.run(default_config())
             feat = featurize(g, p)
-            for c in [default_config()] + sp.random_configs(per_workload, rng):
+            for i, c in enumerate([default_config()] + sp.random_configs(per_workload, rng)):
                 t = td if c == default_config() else drv.run(c)
+                if t * 1e3 > 4 * target_ms:
+                    log("    config %d: %.1f ms" % (i, t * 1e3))
                 if t >= FLOAT_INF:
                     continue
                 X.append(feat(c))
