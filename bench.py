@@ -18,7 +18,9 @@ WRITE_SIZE, calibrated on a copy kernel) and `hbm_frac` their rate against
 `cpu_baseline` times the CPU oracle's whole-graph K4 + K5
 count on the host's cores (at most 16) and checks its counts against the
 GPU's; `naive_c1` times the reference's own CPU path (naive_pm, restated) on
-its triangle workload, one thread.
+its triangle workload, one thread. `hipgraph` times the same step captured
+once into a hipGraph and replayed `--steps` times (gat_count_graph), its
+counts checked against the eager steps'.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -384,6 +386,12 @@ def main():
     if counts != acc_counts:
         raise AssertionError("batched %s and instrumented %s counts differ" % (counts, acc_counts))
     t_b = statistics.median(secs)
+    # the same step captured once into a hipGraph and replayed (gat_count_graph)
+    tg0 = time.perf_counter()
+    g_counts, g_st = batched.count(cfg, graph_replays=args.steps)
+    g_wall = time.perf_counter() - tg0
+    if g_counts != counts:
+        raise AssertionError("hipGraph replay %s and eager %s counts differ" % (g_counts, counts))
     main_ms = statistics.median(mains[0])
     second_ms = statistics.median(mains[1])
     # the roofline is for the launch that dominates the step
@@ -456,6 +464,8 @@ def main():
         "device_ms_plain_same_config": plain_same_ms,
         "batched_vs_plain_same_config": t_plain_same / t_b if t_plain_same else None,
         "roofline": roofline,
+        "hipgraph": {"replays": args.steps, "device_ms_last_replay": g_st["seconds"] * 1e3,
+                     "wall_ms_per_replay_with_capture": g_wall / args.steps * 1e3},
     }
     if args.timeline:
         _, tst = batched.count(cfg, timeline=True)

@@ -155,6 +155,47 @@ __device__ __forceinline__ long long bperm64(long long x, int src) {
     const int lo = bperm((int)(x & 0xffffffffll), src), hi = bperm((int)(x >> 32), src);
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
+// Reads of one lane's registers by the whole wave: LanePull for a lane index
+// that is the same in every lane (v_readlane, no LDS crossbar), PermPull for
+// a per-lane index (ds_bpermute).
+struct LanePull {
+    int s;
+    __device__ __forceinline__ int operator()(int x) const { return __builtin_amdgcn_readlane(x, s); }
+    __device__ __forceinline__ long long operator()(long long x) const {
+        const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), s);
+        const int hi = __builtin_amdgcn_readlane((int)(x >> 32), s);
+        return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+};
+struct PermPull {
+    int s;
+    __device__ __forceinline__ int operator()(int x) const { return bperm(x, s); }
+    __device__ __forceinline__ long long operator()(long long x) const { return bperm64(x, s); }
+};
+// Lanes own consecutive segments [off_l, off_l + len_l) of T elements, off_l
+// not decreasing with l; lane j of a chunk of W elements starting at k0
+// handles element k0 + j, whose owner is the last lane whose segment starts
+// at or before it. The lanes with off_l <= x are a prefix of the wave, so
+// one ballot gives the owner of a position that is the same in every lane:
+// when the chunk's first and last elements have one owner, so do all its
+// elements, and body(LanePull) reads that lane's fields with readlane;
+// otherwise each lane finds its owner by binary search over the lanes and
+// body(PermPull) reads them with ds_bpermute. Every lane must be active.
+template <class F>
+__device__ __forceinline__ void by_owner(int off, int k0, int T, F &&body) {
+    const int s0 = __popcll(ballot(off <= k0)) - 1;
+    const int s1 = __popcll(ballot(off <= min(k0 + W, T) - 1)) - 1;
+    if (s0 == s1) {
+        body(LanePull{s0});
+    } else {
+        const int j = k0 + lane_id();
+        int s = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+            if (bperm(off, s + step) <= j && s + step < W) s += step;
+        body(PermPull{s});
+    }
+}
 // LDS accesses of one wavefront are performed in program order; the fences
 // keep the compiler from moving them across this point.
 __device__ __forceinline__ void wsync() {
@@ -597,12 +638,11 @@ __device__ __noinline__ void cancel_donations(const KArgs &a, WaveState *S, int 
     wsync();
 }
 
-// Split only in the tail: the task has outrun its budget and the queue it
-// came from has nothing left for idle waves.
+// Split only in the tail: the queue the task came from has nothing left for
+// idle waves (queue_drained) and the task has outrun its budget (want_split).
 template <bool GS>
-__device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
+__device__ __forceinline__ bool queue_drained(const KArgs &a) {
     if (a.split_ticks <= 0 || a.sub_mode >= kSplitLevels) return false;
-    if (uni64(wall_clock64()) - S->t0 <= a.split_ticks) return false;
     if (GS) {  // the HBM launch: its overflow list is drained
         const unsigned long long made = __atomic_load_n(a.ovf_n, __ATOMIC_RELAXED);
         const unsigned long long total = made < (unsigned long long)a.ovf_cap ? made : (unsigned long long)a.ovf_cap;
@@ -613,6 +653,12 @@ __device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
     const unsigned long long made = __atomic_load_n(a.in_n, __ATOMIC_RELAXED);
     const unsigned long long total = made < (unsigned long long)a.in_cap ? made : (unsigned long long)a.in_cap;
     return __atomic_load_n(&a.queue[1 + a.sub_mode], __ATOMIC_RELAXED) >= total;
+}
+template <bool GS>
+__device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
+    if (a.split_ticks <= 0 || a.sub_mode >= kSplitLevels) return false;
+    if (uni64(wall_clock64()) - S->t0 <= a.split_ticks) return false;
+    return queue_drained<GS>(a);
 }
 
 // Hands candidates [lo, hi) of the depth-d frame's node to the next split
@@ -748,9 +794,28 @@ __device__ __forceinline__ void leaf(const Ctx &C, const GatNode &N, int d, int 
     unsigned long long bytes = 0;
     for (int cb = cur; cb < hi; cb += W) {
         const int ci = cb + ln;
-        const bool valid = ci < hi;
         int c = 0;
-        if (valid) c = with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[ci]; });
+        if (ci < hi) c = with_slot<GS>(C, N.loop_slot, [&](auto p) { return p[ci]; });
+        // A chunk whose candidates' neighbour lists add up to more elements
+        // than one wave sweeps within the split budget (about 3 ticks of the
+        // 100 MHz clock per element) is not swept whole once the queue is
+        // drained. First the rest of the shallowest loop above the leaf that
+        // has candidates left goes to the next split launch (the most work
+        // for one donation record). With none left, the leaf's candidates
+        // after the first go, in about as many parts as the chunk holds
+        // budgets of work (at most 64), so that whole waves sweep their lists
+        // in parallel; the wave keeps the first candidate.
+        if (can_split && hi - cb > 1 && queue_drained<GS>(a)) {
+            const long long lw = ci < hi ? a.off[c + 1] - a.off[c] : 0;
+            const long long tot = uni64((long long)wave_sum64((unsigned long long)lw));
+            if (3 * tot > a.split_ticks && !split_shallowest<GS>(C, d - 1)) {
+                const long long budgets = 3 * tot / a.split_ticks + 1;
+                const long long np = budgets < W ? budgets : W;
+                const int r = hi - cb - 1;
+                if (donate(C, d, node, cb + 1, hi, (int)((r + np - 1) / np), !GS)) hi = cb + 1;
+            }
+        }
+        const bool valid = ci < hi;
         bool pass = valid && !equals_any(S, N.excl, d, c);
         const bool dup = pass && N.ap_levels && equals_any(S, N.ap_levels, d, c);
         const long long ab = valid ? a.off[c] : 0;
@@ -806,35 +871,32 @@ __device__ __forceinline__ void leaf(const Ctx &C, const GatNode &N, int d, int 
             int cnt = 0;
             for (int k0 = 0; k0 < T; k0 += W) {
                 const int j = k0 + ln;
-                // owner: the last lane whose segment starts at or before j
-                int s = 0;
-#pragma unroll
-                for (int step = 32; step >= 1; step >>= 1)
-                    if (bperm(myoff, s + step) <= j && s + step < W) s += step;
-                const int s_off = bperm(myoff, s);
-                const int s_sa = bperm((int)sa, s);
-                const long long s_ab = bperm64(ab, s);
-                const int s_alen = bperm(alen, s);
-                const int s_blen = bperm(blen, s);
-                const int s_hub = bperm(hub_a, s);
-                const int s_lim = bperm(lim_a, s);
-                const int s_limb = bperm(lim_b, s);
                 bool hit = false;
-                if (j < T) {
-                    const int idx = j - s_off;
-                    int x;
-                    if (s_sa) {
-                        x = a.adj[s_ab + idx];
-                        if (!has_r) hit = true;
-                        else if (b_lds) hit = has(Bl, s_blen, x, a.lin_thr);
-                        else if (hub_b >= 0) hit = hub_has(a, hub_b, x, s_limb);
-                        else hit = has(Bg, s_blen, x, a.lin_thr);
-                    } else {
-                        x = b_lds ? Bl[idx] : Bg[idx];
-                        hit = s_hub >= 0 ? hub_has(a, s_hub, x, s_lim) : has(a.adj + s_ab, s_alen, x, a.lin_thr);
+                by_owner(myoff, k0, T, [&](const auto &pull) {
+                    const int s_off = pull(myoff);
+                    const int s_sa = pull((int)sa);
+                    const long long s_ab = pull(ab);
+                    const int s_alen = pull(alen);
+                    const int s_blen = pull(blen);
+                    const int s_hub = pull(hub_a);
+                    const int s_lim = pull(lim_a);
+                    const int s_limb = pull(lim_b);
+                    if (j < T) {
+                        const int idx = j - s_off;
+                        int x;
+                        if (s_sa) {
+                            x = a.adj[s_ab + idx];
+                            if (!has_r) hit = true;
+                            else if (b_lds) hit = has(Bl, s_blen, x, a.lin_thr);
+                            else if (hub_b >= 0) hit = hub_has(a, hub_b, x, s_limb);
+                            else hit = has(Bg, s_blen, x, a.lin_thr);
+                        } else {
+                            x = b_lds ? Bl[idx] : Bg[idx];
+                            hit = s_hub >= 0 ? hub_has(a, s_hub, x, s_lim) : has(a.adj + s_ab, s_alen, x, a.lin_thr);
+                        }
+                        for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
                     }
-                    for (int q = 0; q < nex; q++) hit &= (x != ex[q]);
-                }
+                });
                 const unsigned long long m = ballot(hit);
                 const int lo = max(myoff - k0, 0), hi2 = min(myoff + mylen - k0, W);
                 if (lo < hi2) cnt += __popcll(m & range_mask(lo, hi2));
@@ -1255,7 +1317,7 @@ constexpr int CBIGW = 32;  // workgroup tier 1: d <= 1024, rows in LDS when it f
 // Workgroup-tier items are (vertex, part) pairs, vertex * CSPLIT_ENC + part:
 // the parts of a vertex share out its pairs (each builds the rows).
 constexpr int CSPLIT_ENC = 16;
-constexpr int CSPLIT12 = 4;   // parts per vertex in tiers 1 and 2
+constexpr int CSPLIT12 = 2;   // parts per vertex in tiers 1 and 2
 constexpr int CSPLIT0 = 1;    // parts per tier-0 vertex with d > CSPLIT0_D
 constexpr int CSPLIT0_D = 256;
 __device__ __forceinline__ int clique_parts(int tier, int d) {
@@ -1318,29 +1380,27 @@ __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first,
         const int myoff = excl_scan(mylen, &T);
         for (int k0 = 0; k0 < T; k0 += W) {
             const int j = k0 + ln;
-            int so = 0;
-#pragma unroll
-            for (int st = 32; st >= 1; st >>= 1)
-                if (bperm(myoff, so + st) <= j && so + st < W) so += st;
             // every lane takes part in the cross-lane reads: a segment's
             // owner lane may itself have no element here
-            const int s_off = bperm(myoff, so);
-            const int s_sx = bperm((int)sx, so);
-            const long long s_xb = bperm64(xb, so);
-            const int s_xl = bperm(xl, so);
-            if (j < T) {
-                const int s_ci = first + (cb + so) * stride;
-                const int idx = j - s_off;
-                int pos = -1;
-                if (s_sx) {
-                    const int y = a.adj[s_xb + idx];
-                    const int p = lbound(L, s_ci, y);
-                    if (p < s_ci && L[p] == y) pos = p;
-                } else {
-                    if (has(a.adj + s_xb, s_xl, L[idx], a.lin_thr)) pos = idx;
+            by_owner(myoff, k0, T, [&](const auto &pull) {
+                const int s_off = pull(myoff);
+                const int s_sx = pull((int)sx);
+                const long long s_xb = pull(xb);
+                const int s_xl = pull(xl);
+                const int s_ci = pull(ci);
+                if (j < T) {
+                    const int idx = j - s_off;
+                    int pos = -1;
+                    if (s_sx) {
+                        const int y = a.adj[s_xb + idx];
+                        const int p = lbound(L, s_ci, y);
+                        if (p < s_ci && L[p] == y) pos = p;
+                    } else {
+                        if (has(a.adj + s_xb, s_xl, L[idx], a.lin_thr)) pos = idx;
+                    }
+                    if (pos >= 0) atomicOr(&rows[(long long)s_ci * w + (pos >> 5)], 1u << (pos & 31));
                 }
-                if (pos >= 0) atomicOr(&rows[(long long)s_ci * w + (pos >> 5)], 1u << (pos & 31));
-            }
+            });
         }
     }
 }

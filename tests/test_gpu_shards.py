@@ -206,14 +206,16 @@ def test_rmat22_batched_31_shards(gat, orc, gpu):
 def test_count_sharded_equals_count(gat, gpu):
     """The sharded driver (many tiny shards) reports what gat_count does, in
     every mode, for patterns whose divisor exceeds 1 too."""
-    from graphautotune_amd.sharded import count_sharded
+    from graphautotune_amd.sharded import count_sharded, edge_shards
     g = gat.Graph.rmat(9, 8, 0.57, 0.19, 0.19, 5)
     multi = [p for p in connected_patterns(5) if gat.Plan(g, [p], "unique").count_info(0)["divisor"] > 1][:3]
     for pats, mode in (([HOUSE, C5], "unique"), (multi, "unique"), (multi, "labeled"), ([K4, HOUSE], "reference")):
         plan = gat.Plan(g, pats, mode)
         got, shards = count_sharded(plan, target_s=1e-4, first=3)
         assert got == plan.count()[0], (pats, mode)
-        assert len(shards) > 3 and shards[-1][1] == g.v_cnt()
+        # array-store plans are walked in directed-edge shards, clique plans in root shards
+        end = int(plan.run_offsets()[-1]) if edge_shards(plan) else g.v_cnt()
+        assert len(shards) > 3 and shards[0][0] == 0 and shards[-1][1] == end
 
 
 def test_reference_mode_shards_match_oracle(gat, orc, gpu):
