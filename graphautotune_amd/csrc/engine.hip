@@ -1329,6 +1329,13 @@ constexpr int CBLOCKW1 = 16;  // waves per workgroup, tiers 1 and 2 (few, heavy 
 // to an odd number so that lanes reading the same word of different rows
 // hit different LDS banks (an even stride folds them onto a few banks).
 __host__ __device__ constexpr int row_words(int d) { return ((d + 31) >> 5) | 1; }
+// LDS index of a workgroup tier's list L (lists of up to lcap vertices):
+// open addressing with linear probing over 2 lcap 16-bit slots (a power of
+// two), each 0 or 1 + the position in L of the vertex stored there.
+__host__ __device__ constexpr int index_slots(int lcap) { return 2 * lcap; }
+__device__ __forceinline__ unsigned index_hash(int y, unsigned mask) {
+    return ((unsigned)y * 0x9E3779B1u >> 16) & mask;
+}
 
 struct CArgs {
     const long long *off;
@@ -1362,11 +1369,12 @@ struct CArgs {
 // zeroed), for the rows p = first, first + stride, first + 2 stride, ...
 // (a workgroup's waves interleave their rows, so light and heavy rows mix):
 // lanes sweep the (row, element) pairs of 64 of these rows at a time; each
-// element of N+(L[p]) is searched in L[0,p), or each L[q], q < p, in
-// N+(L[p]) when that list is gallop_ratio times longer than p.
+// element of N+(L[p]) is searched in L[0,p) (by the LDS index ix of L when
+// given, else by binary search), or each L[q], q < p, in N+(L[p]) when that
+// list is gallop_ratio times longer than p.
 template <bool ACC, class LP, class RP>
 __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first, int stride, int ln,
-                         unsigned long long &bytes) {
+                         unsigned long long &bytes, const unsigned short *ix = nullptr, unsigned ix_mask = 0) {
     for (int cb = 0; first + cb * stride < d; cb += W) {
         const int ci = first + (cb + ln) * stride;
         const bool valid = ci < d;
@@ -1393,8 +1401,19 @@ __device__ void vtx_rows(const CArgs &a, LP L, int d, RP rows, int w, int first,
                     int pos = -1;
                     if (s_sx) {
                         const int y = a.adj[s_xb + idx];
-                        const int p = lbound(L, s_ci, y);
-                        if (p < s_ci && L[p] == y) pos = p;
+                        if (ix) {  // probe the index of L: a slot of 0 ends the probe
+                            for (unsigned h = index_hash(y, ix_mask);; h = (h + 1) & ix_mask) {
+                                const int e = ix[h];
+                                if (e == 0) break;
+                                if (L[e - 1] == y) {
+                                    if (e - 1 < s_ci) pos = e - 1;
+                                    break;
+                                }
+                            }
+                        } else {
+                            const int p = lbound(L, s_ci, y);
+                            if (p < s_ci && L[p] == y) pos = p;
+                        }
                     } else {
                         if (has(a.adj + s_xb, s_xl, L[idx], a.lin_thr)) pos = idx;
                     }
@@ -1668,6 +1687,11 @@ __global__ void __launch_bounds__(64 * NW, NW == CBLOCKW0 ? 8 : 1) clique_block_
     // LDS: the list (then the pair offsets), then the rows
     int *Ls = reinterpret_cast<int *>(smem);
     unsigned *rows_l = reinterpret_cast<unsigned *>(Ls + (LDS ? lcap + 4 : 0));
+    // the index of L after the rows (LDS tiers): one entry per thread
+    static_assert(!LDS || lcap <= 64 * NW, "one index entry per thread");
+    unsigned short *ix = LDS ? reinterpret_cast<unsigned short *>(rows_l + lcap * row_words(lcap)) : nullptr;
+    constexpr unsigned ix_mask = (unsigned)index_slots(lcap) - 1;
+    static_assert((index_slots(lcap) & (index_slots(lcap) - 1)) == 0, "index slots are a power of two");
     __shared__ long long vsh;
     __shared__ int wtot[NW];
     const int hws = row_words(a.hcap);
@@ -1699,13 +1723,34 @@ __global__ void __launch_bounds__(64 * NW, NW == CBLOCKW0 ? 8 : 1) clique_block_
                 for (int k = threadIdx.x; k < d; k += 64 * NW) Ls[k] = a.adj[rb + k];
             unsigned *rows = LDS ? rows_l : rows_g;
             for (long long k = threadIdx.x; k < (long long)d * w; k += 64 * NW) rows[k] = 0u;
+            if (LDS)
+                for (int k = threadIdx.x; k < index_slots(lcap); k += 64 * NW) ix[k] = 0;
             // every part builds the rows; W_alg charges one build
             unsigned long long build_bytes = 0;
             if (ACC && threadIdx.x == 0) build_bytes += 4ull * d + 16ull;
             if (!LDS) __threadfence();
             __syncthreads();
+            if (LDS) {
+                // thread i enters L[i]: each round it writes 1 + i to its slot
+                // if the slot is empty, and after a barrier moves on if
+                // another thread's entry holds the slot. Entries are never
+                // overwritten, and one moves past a slot only when that slot
+                // is taken, so every slot between an entry's hash and its
+                // place stays full, as linear probing needs.
+                const int i = threadIdx.x;
+                bool pending = i < d;
+                unsigned h = pending ? index_hash(Ls[i], ix_mask) : 0u;
+                while (__syncthreads_or(pending)) {
+                    if (pending && ix[h] == 0) ix[h] = (unsigned short)(i + 1);
+                    __syncthreads();
+                    if (pending) {
+                        if (ix[h] == i + 1) pending = false;
+                        else h = (h + 1) & ix_mask;
+                    }
+                }
+            }
             const int *L = LDS ? Ls : a.adj + rb;
-            vtx_rows<ACC>(a, L, d, rows, w, wid, NW, ln, build_bytes);
+            vtx_rows<ACC>(a, L, d, rows, w, wid, NW, ln, build_bytes, ix, ix_mask);
             if (!LDS) __threadfence();
             __syncthreads();
             // pair offsets: block-wide exclusive prefix of popc(M_p)
@@ -2057,10 +2102,12 @@ static int clique_shape(const DeviceInfo &di, const gat_kernel_config &cfg, Laun
     L->clique_lds = size_t(L->clique_wave_bytes) * cfg.block_waves;
     if (L->clique_lds > di.lds_per_block)
         GAT_FAIL(GAT_EINFEASIBLE, "bitmap config needs %zu B of LDS per workgroup", L->clique_lds);
-    L->mid_lds = (size_t)4 * (32 * gatk::CMIDW + 4) + (size_t)4 * 32 * gatk::CMIDW * gatk::row_words(32 * gatk::CMIDW);
+    L->mid_lds = (size_t)4 * (32 * gatk::CMIDW + 4) + (size_t)4 * 32 * gatk::CMIDW * gatk::row_words(32 * gatk::CMIDW) +
+                 (size_t)2 * gatk::index_slots(32 * gatk::CMIDW);
     if (L->mid_lds > di.lds_per_block) GAT_FAIL(GAT_EINFEASIBLE, "device LDS too small for the clique block tier");
     L->mid_grid = dim3(std::max(1, (int)(di.lds_per_cu / L->mid_lds)) * di.cus);
-    const size_t t1 = (size_t)4 * (32 * gatk::CBIGW + 4) + (size_t)4 * 32 * gatk::CBIGW * gatk::row_words(32 * gatk::CBIGW);
+    const size_t t1 = (size_t)4 * (32 * gatk::CBIGW + 4) + (size_t)4 * 32 * gatk::CBIGW * gatk::row_words(32 * gatk::CBIGW) +
+                      (size_t)2 * gatk::index_slots(32 * gatk::CBIGW);
     L->t1_lds = t1 <= di.lds_per_block ? t1 : 0;
     if (L->t1_lds) L->t1_grid = dim3(std::max(1, (int)(di.lds_per_cu / t1)) * di.cus);
     return GAT_OK;
