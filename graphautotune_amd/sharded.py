@@ -257,7 +257,8 @@ def main(argv=None):
                          "last root under S on; the roots before it are reported as excluded")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="count only roots [LO, HI) and report raw sums (add the raw sums of a partition, "
-                         "then divide by the divisor and multiply by the multiplier)")
+                         "then divide by the divisor and multiply by the multiplier); with --estimate, "
+                         "sample only those roots' units")
     args = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
@@ -279,7 +280,11 @@ def main(argv=None):
     if args.estimate:
         head, probes = 0, []
         n_units, _, _, first_unit = unit_space(plan, cfg)
-        if args.head_limit is not None:
+        r_hi = plan.graph.v_cnt()
+        if args.roots:  # only the units of the roots [LO, HI)
+            head, r_hi = (int(x) for x in args.roots.split(":"))
+            n_units = first_unit(r_hi) if r_hi < plan.graph.v_cnt() else n_units
+        elif args.head_limit is not None:
             probes, head = head_ladder(plan, cfg, args.head_limit, log=log)
         est = stratified_estimate(plan, cfg, per_stratum=args.per_stratum, min_block_s=args.min_block,
                                   seed=args.seed, log=log, units=(first_unit(head), n_units))
@@ -288,12 +293,15 @@ def main(argv=None):
         print(json.dumps({
             "graph": args.graph, "vertices": g.v_cnt(), "undirected_edges": E, "patterns": pats, "mode": args.mode,
             "kernel_config": cfg, "schedule_ranks": sched.get("schedule_rank"), "per_stratum": args.per_stratum,
-            "min_block_s": args.min_block, "seed": args.seed, "head_limit_s": args.head_limit, "head_probes": probes, "excluded_roots": [0, head],
+            "min_block_s": args.min_block, "seed": args.seed, "head_limit_s": args.head_limit,
+            "head_probes": probes, "roots": [head, r_hi], "excluded_roots": [0, head] if not args.roots else None,
+            "raw": est["raw"], "raw_se": est["raw_se"],
             "excluded_units": [0, first_unit(head)], "units": n_units, "unit": est["unit"],
             "estimated_counts": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw"], infos)],
             "estimated_counts_se": [r / i["divisor"] * i["multiplier"] for r, i in zip(est["raw_se"], infos)],
             "projected_device_seconds": est["seconds"], "projected_device_seconds_se": est["seconds_se"],
-            "projected_edges_per_s": E / est["seconds"] if est["seconds"] > 0 and head == 0 else None,
+            "projected_edges_per_s": E / est["seconds"] if est["seconds"] > 0 and head == 0 and not args.roots
+            else None,
             "projected_units_per_s": (n_units - first_unit(head)) / est["seconds"] if est["seconds"] > 0 else None,
             "sampled_device_seconds": sum(st["sampled_seconds"] for st in est["strata"]),
             "strata": est["strata"]}), flush=True)
