@@ -115,6 +115,11 @@ __device__ __forceinline__ void tl_write(long long *tl, int idx, int launch, int
     r[7] = pairs;
 }
 
+// Subtask blocks a task may hand on between commits. A batched plan's trie
+// starts a loop for every child node of a candidate, and once a task is due
+// to split each such loop goes to the next split launch with one record.
+constexpr int kMaxDonations = 64;
+
 struct WaveState {
     unsigned long long task_acc[GATP_MAX_TERMS];
     unsigned long long wave_acc[GATP_MAX_TERMS];
@@ -131,8 +136,8 @@ struct WaveState {
     long long t0;               // wall clock at the start of the task
     long long rb;               // first edge of the task's root v0
     long long sub_edge;         // root edge of the running subtask, or -1
-    long long don_base[8];      // subtask blocks donated since the last commit
-    int don_cnt[8];
+    long long don_base[kMaxDonations];  // subtask blocks donated since the last commit
+    int don_cnt[kMaxDonations];
     int n_don;
     int base_depth;             // frames exist from this depth on (2, or a subtask's depth)
     int path_pos[GATP_MAX_DEPTH + 1];  // a subtask's positions at depths 2 .. base_depth-1
@@ -670,7 +675,7 @@ __device__ __forceinline__ bool want_split(const KArgs &a, const WaveState *S) {
 __device__ __noinline__ bool donate(const Ctx C, int d, int node, int lo, int hi, int min_chunk, bool record) {
     const KArgs &a = C.a;
     WaveState *S = C.S;
-    if (record && S->n_don >= 8) return false;
+    if (record && S->n_don >= kMaxDonations) return false;
     const long long edge = S->sub_edge >= 0 ? S->sub_edge : S->rb + (S->f_cur[1] - 1);
     const int r = hi - lo;
     const int chunk = max(min_chunk, (r + W - 1) / W);
