@@ -22,10 +22,14 @@ class EngineBug(RuntimeError):
 
 class Driver:
     """roots=(lo, hi) times a root shard (gat_count_shard, raw sums) instead
-    of the whole graph: on large graphs a fixed shard is the workload."""
+    of the whole graph: on large graphs a fixed shard is the workload.
+    plan_options are passed to core.Plan (e.g. schedule="ranked" with
+    per-pattern schedule_rank)."""
 
-    def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1, roots=None):
-        self.plan = core.Plan(graph, patterns, mode)
+    def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1, roots=None, plan_options=None,
+                 report=None):
+        self.plan = core.Plan(graph, patterns, mode, **(plan_options or {}))
+        self.report = report  # called with a line per measured config
         self.graph = graph
         self.patterns = list(patterns)
         self.reps, self.warmup = reps, warmup
@@ -60,6 +64,8 @@ class Driver:
             raise
         t = statistics.median(secs)
         self.log.append((cfg, t))
+        if self.report:
+            self.report("measured %.3f ms %s" % (t * 1e3, cfg))
         return t
 
     def batch_run(self, configs):

@@ -12,7 +12,7 @@ import logging
 import os
 import shutil
 
-from .. import core
+from .. import core, workloads
 from .driver import Driver
 from .manipulator import Manipulator
 from .modeling import Modeling
@@ -36,7 +36,10 @@ def load_graph(spec):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("data")
-    ap.add_argument("pattern", help="pattern string, or several separated by commas (batched)")
+    ap.add_argument("pattern", help="pattern string, or several separated by commas (batched), or 'batched' "
+                                    "for workloads.batched_patterns()")
+    ap.add_argument("--schedule-ranks", default=None, metavar="JSON",
+                    help="a result of tuning.batched_study: tune the plan with its per-pattern Schedule ranks")
     ap.add_argument("-r", "--run_mode", default="tune", choices=["tune", "run_default", "run_best"])
     ap.add_argument("-c", "--clear", action="store_true")
     ap.add_argument("--results", default="tuning_results")
@@ -49,8 +52,15 @@ def main(argv=None):
         shutil.rmtree(args.results)
     os.makedirs(args.results, exist_ok=True)
     g = load_graph(args.data)
-    patterns = args.pattern.split(",")
-    driver = Driver(g, patterns, "unique", reps=1, warmup=0)
+    patterns = workloads.batched_patterns() if args.pattern == "batched" else args.pattern.split(",")
+    opts = None
+    if args.schedule_ranks:
+        with open(args.schedule_ranks) as f:
+            tuned = json.load(f)
+        if [core.Pattern(p).adj for p in tuned["patterns"]] != [core.Pattern(p).adj for p in patterns]:
+            ap.error("%s holds Schedule ranks for other patterns" % args.schedule_ranks)
+        opts = {"schedule": "ranked", "schedule_rank": list(tuned["ranks"])}
+    driver = Driver(g, patterns, "unique", reps=1, warmup=0, plan_options=opts, report=logger.info)
     feat = featurizer(g, patterns)
     model = Modeling(feat, os.path.join(args.results, "record.json"))
     if args.run_mode == "run_default":
