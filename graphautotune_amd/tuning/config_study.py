@@ -9,9 +9,12 @@ judge, and the model is compared with picking `budget` configs at random
 Each (graph, pattern) workload is a fixed root shard (gat_count_shard) of
 mid-degree roots, grown until the default config takes `target_ms`, so that
 a 6-vertex pattern on a scale-21 graph is measured in milliseconds rather
-than hours; every config of a workload must give the same raw sums. The
-model regresses log(runtime / default runtime) of its workload, which is
-what ranking within a workload needs.
+than hours; every config of a workload must give the same raw sums. A
+config whose count of the shard's first eighth already takes 2.5 times the
+default's whole shard is scored by that time scaled to the shard (Driver
+probe): it is far from the best either way, and its full run could take
+minutes. The model regresses log(runtime / default runtime) of its
+workload, which is what ranking within a workload needs.
 
     python -m graphautotune_amd.tuning.config_study --out results/config_study_rmat16-20_to_rmat21.json
     python -m graphautotune_amd.tuning.config_study --judge-clique --out results/clique_grid_rmat20.json
@@ -113,11 +116,11 @@ def collect(scales, patterns_per_scale, train_pool, per_workload, target_ms, see
                 continue
             drv = Driver(g, [p], "unique", reps=1, warmup=1, roots=(lo, hi))
             td = drv.run(default_config())
+            drv.probe = (max(1, (hi - lo) // 8), 2.5 * td)
+            drv.report = log
             feat = featurize(g, p)
-            for i, c in enumerate([default_config()] + sp.random_configs(per_workload, rng)):
+            for c in [default_config()] + sp.random_configs(per_workload, rng):
                 t = td if c == default_config() else drv.run(c)
-                if t * 1e3 > 4 * target_ms:
-                    log("    config %d: %.1f ms" % (i, t * 1e3))
                 if t >= FLOAT_INF:
                     continue
                 X.append(feat(c))
@@ -159,6 +162,8 @@ def rank(model, scale, patterns, pool_size, budget, target_ms, seed=0, log=print
             continue
         drv = Driver(g, [p], "unique", reps=reps, warmup=1, roots=(lo, hi))
         td = drv.run(default_config())
+        drv.probe = (max(1, (hi - lo) // 8), 2.5 * td)
+        drv.report = log
         truth = np.array([td] + [drv.run(c) for c in pool[1:]])
         ok = truth < FLOAT_INF
         feat = featurize(g, p)

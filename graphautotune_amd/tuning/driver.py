@@ -24,7 +24,12 @@ class Driver:
     """roots=(lo, hi) times a root shard (gat_count_shard, raw sums) instead
     of the whole graph: on large graphs a fixed shard is the workload.
     plan_options are passed to core.Plan (e.g. schedule="ranked" with
-    per-pattern schedule_rank)."""
+    per-pattern schedule_rank). probe=(k, limit_s), with roots: each config
+    first counts the shard's first k roots once, and when that takes limit_s
+    or more the config is scored by that time scaled to the shard's size
+    instead of being run whole (a config tens of times slower than the
+    default tells a ranker all it needs, and its full run could take
+    minutes); such scores are marked in the report."""
 
     def __init__(self, graph, patterns, mode="unique", reps=3, warmup=1, roots=None, plan_options=None,
                  report=None):
@@ -34,7 +39,9 @@ class Driver:
         self.patterns = list(patterns)
         self.reps, self.warmup = reps, warmup
         self.roots = roots
+        self.probe = None
         self.counts = None
+        self.probe_counts = None
         self.log = []
 
     def _count(self, cfg):
@@ -45,6 +52,20 @@ class Driver:
     def run(self, config):
         cfg = config.as_dict() if hasattr(config, "as_dict") else dict(config)
         try:
+            if self.probe and self.roots and self.roots[1] - self.roots[0] > self.probe[0]:
+                k, limit = self.probe
+                lo, hi = self.roots
+                c, st = self.plan.count_shard(lo, lo + k, cfg)
+                if self.probe_counts is None:
+                    self.probe_counts = c
+                elif c != self.probe_counts:
+                    raise AssertionError("config %s changed the counts: %s vs %s" % (cfg, c, self.probe_counts))
+                if st["seconds"] >= limit:
+                    t = st["seconds"] * (hi - lo) / k
+                    self.log.append((cfg, t))
+                    if self.report:
+                        self.report("scaled from %d of %d roots: %.3f ms %s" % (k, hi - lo, t * 1e3, cfg))
+                    return t
             for _ in range(self.warmup):
                 self._count(cfg)
             secs = []
