@@ -7,9 +7,10 @@ judge, and the model is compared with picking `budget` configs at random
 (the method without it) and with the default config.
 
 Each (graph, pattern) workload is a fixed root shard (gat_count_shard) of
-mid-degree roots, grown until the default config takes `target_ms`, so that
-a 6-vertex pattern on a scale-21 graph is measured in milliseconds rather
-than hours; every config of a workload must give the same raw sums. A
+mid-degree roots, grown from one root until the default config takes
+`target_ms` (a workload whose one root takes over 4 target_ms is left out),
+so that a 6-vertex pattern on a scale-21 graph is measured in milliseconds
+rather than hours; every config of a workload must give the same raw sums. A
 config whose count of the shard's first eighth already takes 2.5 times the
 default's whole shard is scored by that time scaled to the shard (Driver
 probe): it is far from the best either way, and its full run could take
@@ -60,7 +61,7 @@ def rmat(scale):
     return core.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, scale)
 
 
-def pick_shard(graph, pattern, target_ms, first=64, log=None):
+def pick_shard(graph, pattern, target_ms, first=1, log=None):
     """Roots [lo, lo + size) of the degree-ordered graph, lo = n / 16
     (mid-degree roots), size doubled from `first` until the default config
     takes target_ms or the shard reaches the last root, or halved while it
@@ -114,6 +115,9 @@ def collect(scales, patterns_per_scale, train_pool, per_workload, target_ms, see
             if tp * 1e3 < target_ms / 4:
                 log("collect s%d %s: %.2f ms on all roots from %d, too small to time; left out" % (sc, p, tp * 1e3, lo))
                 continue
+            if tp * 1e3 > 4 * target_ms:
+                log("collect s%d %s: %.2f ms on one root, too heavy to time many configs; left out" % (sc, p, tp * 1e3))
+                continue
             drv = Driver(g, [p], "unique", reps=1, warmup=1, roots=(lo, hi))
             td = drv.run(default_config())
             drv.probe = (max(1, (hi - lo) // 8), 2.5 * td)
@@ -159,6 +163,9 @@ def rank(model, scale, patterns, pool_size, budget, target_ms, seed=0, log=print
         pool = [default_config()] + sp.random_configs(pool_size, rng)
         if t0 * 1e3 < target_ms / 4:
             log("rank s%d %s: %.2f ms on all roots from %d, too small to time; left out" % (scale, p, t0 * 1e3, lo))
+            continue
+        if t0 * 1e3 > 4 * target_ms:
+            log("rank s%d %s: %.2f ms on one root, too heavy to time many configs; left out" % (scale, p, t0 * 1e3))
             continue
         drv = Driver(g, [p], "unique", reps=reps, warmup=1, roots=(lo, hi))
         td = drv.run(default_config())
