@@ -61,32 +61,46 @@ def rmat(scale):
     return core.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, scale)
 
 
-def pick_shard(graph, pattern, target_ms, first=1, log=None):
-    """Roots [lo, lo + size) of the degree-ordered graph, lo = n / 16
-    (mid-degree roots), size doubled from `first` until the default config
-    takes target_ms or the shard reaches the last root, or halved while it
-    takes more than twice target_ms (down to one root); returns (lo, hi,
-    default seconds)."""
-    n = graph.v_cnt()
-    lo = n // 16
-    size = first
-    plan = core.Plan(graph, [pattern], "unique")
+def grow_shard(plan, lo, n, target_ms, first=1):
+    """Roots [lo, lo + size): size doubled (or multiplied by 8 while far
+    below target) from `first` until the default config takes target_ms or
+    the shard reaches the last root; returns (hi, default seconds)."""
     base = dict(FIXED)
-    grown = False
+    size = first
     while True:
         hi = min(n, lo + size)
         plan.count_shard(lo, hi, base)
         _, st = plan.count_shard(lo, hi, base)
         ms = st["seconds"] * 1e3
-        if ms > 2 * target_ms and size > 1 and not grown:
-            size //= 2
-            continue
-        if ms >= target_ms or hi == n or (size == 1 and not grown) or ms > 2 * target_ms:
+        if ms >= target_ms or hi == n:
+            return hi, st["seconds"]
+        size *= 2 if ms > target_ms / 8 else 8
+
+
+def pick_shard(graph, pattern, target_ms, first=1, log=None):
+    """A root shard [lo, hi) of the degree-ordered graph whose default-config
+    count takes about target_ms (grow_shard), starting at lo = n / 16
+    (mid-degree roots). When every root from lo on together takes under a
+    quarter of the target, lo moves to heavier roots (lo / 4); when one root
+    alone takes over four times the target, to lighter ones (4 lo).
+    Returns (lo, hi, default seconds)."""
+    n = graph.v_cnt()
+    plan = core.Plan(graph, [pattern], "unique")
+    lo, tried = n // 16, set()
+    while True:
+        tried.add(lo)
+        hi, t = grow_shard(plan, lo, n, target_ms, first)
+        ms = t * 1e3
+        nxt = lo
+        if ms < target_ms / 4 and hi == n and lo > 0:
+            nxt = lo // 4
+        elif ms > 4 * target_ms and hi - lo == first and lo < n - 1:
+            nxt = min(n - 1, 4 * lo)
+        if nxt == lo or nxt in tried:
             if log:
                 log("  shard [%d, %d): default %.2f ms" % (lo, hi, ms))
-            return lo, hi, st["seconds"]
-        grown = True
-        size *= 2 if ms > target_ms / 8 else 8
+            return lo, hi, t
+        lo = nxt
 
 
 def featurize(graph, pattern):
