@@ -2124,6 +2124,9 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     if (rc) return rc;
     gat_kernel_config cfg{};
     if (cfg_in) cfg = *cfg_in;
+    // the array store's waves wait on dependent list reads: its default grid
+    // keeps twice the clique store's waves per CU (16 against 8)
+    if (cfg.waves_per_cu <= 0 && (p->cp.clique_ks == 0 || cfg.vertex_store == 1)) cfg.waves_per_cu = 16;
     fill_defaults(&cfg);
     if (cfg.block_waves > 4) GAT_FAIL(GAT_EINFEASIBLE, "block_waves %d > 4", cfg.block_waves);
     Graph &g = *p->graph;

@@ -118,7 +118,8 @@ typedef struct gat_schedule_data {
  * pre-instantiated. Zero in any field selects the engine default. */
 typedef struct gat_kernel_config {
     int32_t block_waves;      /* waves per workgroup (threads_per_block/64)   */
-    int32_t waves_per_cu;     /* persistent grid = waves_per_cu * CUs          */
+    int32_t waves_per_cu;     /* persistent grid = waves_per_cu * CUs (0: 16
+                                 for the array store, 8 for the clique store) */
     int32_t queue_chunk;      /* directed edges taken per work-queue grab      */
     int32_t slot_capacity;    /* int32 entries per LDS prefix-set slot         */
     int32_t leaf_mode;        /* 0 flattened, else lanes per candidate 1..64
