@@ -65,8 +65,8 @@ def raw_sharded(plan, config=None, target_s=5.0, first=1, roots=None, log=None, 
         raw = [a + b for a, b in zip(raw, got)]
         shards.append((lo, hi, st["seconds"]))
         if log and st["seconds"] >= quiet_below:
-            log("shard [%d, %d) %s: %.3f s, split tasks %d; %.1f %% done"
-                % (lo, hi, unit, st["seconds"], st["split_tasks"], 100.0 * (hi - start) / max(1, n - start)))
+            log("shard [%d, %d) %s: %.3f s, split tasks %d; %.1f %% done; raw sums so far %s"
+                % (lo, hi, unit, st["seconds"], st["split_tasks"], 100.0 * (hi - start) / max(1, n - start), raw))
         t = max(st["seconds"], 1e-4)
         size = max(1, int(size * min(4.0, max(0.25, target_s / t))))
         lo = hi
