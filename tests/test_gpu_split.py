@@ -18,7 +18,9 @@ pytestmark = pytest.mark.gpu
 
 SPLIT = {"split_us": 1, "vertex_store": 1}
 NOSPLIT = {"split_us": -1, "vertex_store": 1}
-OVERFLOW = {"slot_capacity": 8, "queue_chunk": 1}
+# 8 waves per CU: the launch shape under which edges overflow in the main
+# launch before they split, so that the HBM launch has tasks to hand on
+OVERFLOW = {"slot_capacity": 8, "queue_chunk": 1, "waves_per_cu": 8}
 
 
 def say(*a):
