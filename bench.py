@@ -78,6 +78,9 @@ def clique_roofline(kernel, ms, walg, prof):
             out["traffic"] = hbm
             out["hbm_achieved_gbps"] = hbm / (ms * 1e-3) / 1e9
             out["hbm_frac"] = out["hbm_achieved_gbps"] / HBM_PEAK_GBPS
+        low = prof.get("hbm_bytes_per_launch_unscaled_fetch", {}).get(kernel)
+        if low:
+            out["hbm_frac_unscaled_fetch"] = low / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS
         d = prof.get("derived", {}).get(kernel)
         if d:
             out["counters"] = {k: d[k] for k in ("valu_issue", "valu_lane_util", "lds_util", "lds_conflict_share",
@@ -96,6 +99,30 @@ def hbm_roofline(kernel, ms, walg, prof):
             out["traffic"] = hbm
             out["hbm_frac"] = hbm / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS
     return out
+
+
+# launch groups of the array store's count: the main launch, and the split
+# subtask levels with the HBM overflow launch and its subtasks
+IEP_GROUPS = {"gatk::engine_kernel<false>": ("gatk::engine_kernel<false>",),
+              "split and HBM launches": ("gatk::engine_sub_kernel<false>", "gatk::engine_kernel<true>",
+                                         "gatk::engine_sub_kernel<true>")}
+
+
+def iep_traffic(prof, group, key="hbm_bytes_per_launch"):
+    """HBM bytes per count of a launch group (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE per dispatch, calibrated, times the group's dispatches per
+    count; key "hbm_bytes_per_launch_unscaled_fetch" for the lower bound),
+    the group's longest kernel and that kernel's counters."""
+    calls = {k["kernel"]: k["calls"] for k in prof["kernel_stats"]}
+    total = {k["kernel"]: k["total_ms"] for k in prof["kernel_stats"]}
+    counts = calls.get("gatk::engine_kernel<false>")
+    hb = prof.get(key, {})
+    names = IEP_GROUPS[group]
+    if not counts or not all(n in hb for n in names if calls.get(n)):
+        return None, None, None
+    traffic = sum(hb[n] * calls[n] for n in names if calls.get(n)) / counts
+    longest = max(names, key=lambda n: total.get(n, 0.0))
+    return traffic, longest, prof.get("derived", {}).get(longest)
 
 
 def log(*a):
@@ -282,13 +309,15 @@ def iep_line(gat, args):
                 ("split and HBM launches", statistics.median(seconds), acc["alg_bytes_second"])]
     name, ms, walg = max(launches, key=lambda x: x[1])
     achieved = walg / (ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_iep.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tr = json.load(f).get("%s-%s" % (args.graph, args.iep), {})
-        if tr.get("config") == cfg:
-            traffic = tr.get("bytes_per_launch", {}).get(name)
+    traffic, traffic_low, counters = None, None, None
+    prof = load_profile("engine_iep_%s_%s.json" % (args.graph, args.iep), cfg)
+    if prof and prof.get("workload") == desc:
+        traffic, longest, d = iep_traffic(prof, name)
+        traffic_low = iep_traffic(prof, name, "hbm_bytes_per_launch_unscaled_fetch")[0]
+        if d:
+            counters = {k: d[k] for k in ("valu_issue", "valu_lane_util", "lds_util", "lds_conflict_share",
+                                          "wait_share", "waves_per_cu") if k in d}
+            counters["kernel"] = longest
     print(json.dumps({
         "metric": "exact-count throughput (edges/s), 5-vertex IEP pattern, array store" +
                   ("; edges = directed edges of the shard's roots" if roots else ""),
@@ -301,6 +330,10 @@ def iep_line(gat, args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "hbm_achieved": traffic / (ms * 1e-3) / 1e9 if traffic else None,
+                     "hbm_frac": traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None,
+                     "hbm_frac_unscaled_fetch": traffic_low / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic_low
+                     else None,
+                     "counters": counters,
                      "alg_bytes_per_launch": walg, "kernel": name, "kernel_ms": ms,
                      "launches_ms": {k: v for k, v, _ in launches},
                      "step_alg_bytes": acc["alg_bytes"], "step_achieved": acc["alg_bytes"] / t / 1e9},

@@ -21,7 +21,11 @@ GRBM_GUI_ACTIVE / 8):
                      spent waiting (dependencies, memory, barriers)
   lds_load_bytes     64 x SQ_INSTS_LDS_LOAD_BANDWIDTH
 HBM bytes per dispatch: FETCH_SIZE and WRITE_SIZE (KiB) times the factor
-that makes a plain copy of known size read and write exactly its size.
+that makes a plain copy of known size read and write exactly its size
+(FETCH_SIZE counts 64 B per read request, and the copy's requests are
+128 B, so its factor is 2). A kernel of scattered small reads issues
+smaller requests, for which FETCH_SIZE is already right: its scaled figure
+is an upper bound, and the unscaled one (also written) a lower bound.
 """
 import argparse
 import collections
@@ -125,6 +129,11 @@ def main(argv=None):
         out["fetch_scale"], out["write_scale"] = fscale, wscale
         out["hbm_bytes_per_launch"] = {k: 1024.0 * (f[k]["FETCH_SIZE"] * fscale + w.get(k, {}).get("WRITE_SIZE", 0)
                                                    * wscale) for k in f if "FETCH_SIZE" in f[k]}
+        # the same without the fetch scale: a lower bound for kernels whose
+        # scattered reads issue requests smaller than the copy kernel's
+        out["hbm_bytes_per_launch_unscaled_fetch"] = {
+            k: 1024.0 * (f[k]["FETCH_SIZE"] + w.get(k, {}).get("WRITE_SIZE", 0) * wscale)
+            for k in f if "FETCH_SIZE" in f[k]}
     with open(args.out, "w") as fh:
         json.dump(out, fh, indent=1)
 
