@@ -1543,6 +1543,34 @@ __device__ void vtx_pairs(const CArgs &a, RP rows, int d, int w, const int *off,
         acc[4] += c4;
         if (a.kmax < 5) continue;
         unsigned c5 = 0;  // at most |t| * d bits: fits 32 bits
+        if (a.kmax == 5) {
+            // the next r's row words are read before the current r's are
+            // used, so the LDS latency overlaps the bit operations
+            for (int rw = 0; rw < wq; rw++) {
+                unsigned br = rw < TC ? pick(tc, rw) : (rp[rw] & rq[rw]);
+                if (!br) continue;
+                int r = rw * 32 + __ffs(br) - 1;
+                br &= br - 1;
+                RP rr = rows + __umul24(r, w);
+                unsigned x0 = rr[0], x1 = rr[1];
+                for (;;) {
+                    const bool more = br != 0;
+                    const int rn = more ? rw * 32 + __ffs(br) - 1 : r;
+                    br &= br - 1;
+                    const RP rnp = rows + __umul24(rn, w);
+                    const unsigned y0 = rnp[0], y1 = rnp[1];
+                    if (ACC) bytes += 8ull * (rw + 1);
+                    c5 += __popc(tc[0] & x0) + __popc(tc[1] & x1);
+                    if (rw >= TC) c5 += and3_popc<MAXW, TC>(rp, rq, rr, rw + 1);
+                    if (!more) break;
+                    rr = rnp;
+                    x0 = y0;
+                    x1 = y1;
+                }
+            }
+            acc[5] += c5;
+            continue;
+        }
         for (int rw = 0; rw < wq; rw++) {
             // every r in word rw of t needs words 0 .. rw (the same count
             // for all lanes still in the loop)
