@@ -277,6 +277,39 @@ class _Shard:
         return self.plan.count_shard(self.roots[0], self.roots[1], cfg, account=account, timeline=timeline)
 
 
+def iep_cpu_reference(gat, g, args):
+    """The reference's own algorithm on the CPU beside the IEP line: the
+    oracle's restatement of the reference engine (engine_kernel.cuh with its
+    IEP tail; reference mode, one thread) over the same roots of the
+    degree-ordered graph, timed, and its value checked against the GPU's
+    reference-mode count of the same roots of the same graph."""
+    from oracle import api as orc
+    off, adj = g.csr()
+    ro, ra = orc.relabel(off, adj)
+    lo, hi = (int(x) for x in args.roots.split(":")) if args.roots else (0, g.v_cnt())
+    pat = IEP_PATTERNS[args.iep]
+    gr = gat.Graph.from_csr(ro, ra)
+    plan = gat.Plan(gr, [pat], "reference")
+    plan.count_shard(lo, hi, {})
+    got, st = plan.count_shard(lo, hi, {})
+    from graphautotune_amd._lib import ScheduleData
+    sd = orc.schedule(pat, ScheduleData)
+    model, _, _ = host_cpu()
+    t0 = time.perf_counter()
+    want, ok = orc.reference_count_roots(ro, ra, sd, lo, hi)
+    secs = time.perf_counter() - t0
+    if not ok or got[0] != want:
+        raise AssertionError("reference-mode roots [%d, %d): GPU %d, oracle %d (defined %s)" % (lo, hi, got[0], want, ok))
+    edges = int(ro[hi] - ro[lo])
+    log("cpu reference path: roots [%d, %d), %d edges, value %d in %.1f s on one thread (%s); GPU %.2f ms"
+        % (lo, hi, edges, want, secs, model, st["seconds"] * 1e3))
+    return {"value": edges / secs, "unit": "edges/s", "cores": 1, "kind": "port",
+            "sample": "the oracle's restatement of the reference engine (reference mode, IEP tail) over roots "
+                      "[%d, %d) of the degree-ordered graph (%d directed edges); value equal to the GPU's "
+                      "reference-mode count of the same roots" % (lo, hi, edges),
+            "seconds": secs, "reference_value": want, "gpu_reference_ms": st["seconds"] * 1e3, "cpu_model": model}
+
+
 def iep_line(gat, args):
     """Auxiliary line: one 5-vertex IEP pattern on the array store (default
     config, hub bitmaps on), unique mode. Roofline of the longer of the main
@@ -318,6 +351,7 @@ def iep_line(gat, args):
             counters = {k: d[k] for k in ("valu_issue", "valu_lane_util", "lds_util", "lds_conflict_share",
                                           "wait_share", "waves_per_cu") if k in d}
             counters["kernel"] = longest
+    cpu = iep_cpu_reference(gat, g, args) if args.cpu_reference else None
     print(json.dumps({
         "metric": "exact-count throughput (edges/s), 5-vertex IEP pattern, array store" +
                   ("; edges = directed edges of the shard's roots" if roots else ""),
@@ -326,7 +360,7 @@ def iep_line(gat, args):
         "data": "synthetic",
         "config": {"workload": "%s; exact unique %s count" % (desc, args.iep), "vertices": g.v_cnt(),
                    "undirected_edges": E, "pattern": IEP_PATTERNS[args.iep], "kernel_config": cfg},
-        "counts": counts, "device_ms": t * 1e3, "overflow_tasks": st["overflow_tasks"],
+        "counts": counts, "device_ms": t * 1e3, "overflow_tasks": st["overflow_tasks"], "cpu_baseline": cpu,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "hbm_achieved": traffic / (ms * 1e-3) / 1e9 if traffic else None,
@@ -359,6 +393,9 @@ def main():
     ap.add_argument("--iep", choices=sorted(IEP_PATTERNS), default=None,
                     help="auxiliary line: one 5-vertex IEP pattern on the array store instead of K4 + K5")
     ap.add_argument("--graph", default="rmat14", help="with --iep: rmatS or clN (see make_graph)")
+    ap.add_argument("--cpu-reference", action="store_true",
+                    help="with --iep: also time the oracle's restatement of the reference engine on the same "
+                         "roots (one thread; minutes per hub-adjacent root)")
     ap.add_argument("--roots", default=None, metavar="LO:HI",
                     help="with --iep: count only the roots [LO, HI) of the degree-ordered graph (a shard)")
     args = ap.parse_args()
