@@ -2180,10 +2180,10 @@ static int prepare(gat_plan *p, const gat_kernel_config *cfg_in, Launch *L) {
     const bool clique = p->cp.clique_ks != 0 && cfg.vertex_store != 1;
     if (!clique && !d->gstore) {
         // HBM set buffers of the overflow launch, where every set fits in the
-        // maximum degree: 8 waves per CU hide the latency of HBM-resident
+        // maximum degree: 16 waves per CU hide the latency of HBM-resident
         // sets, within 4 GiB of buffers.
         const size_t per_wave = (size_t)std::max(1, pd.n_store) * (size_t)std::max(1, g.max_deg) * 4;
-        int waves = std::max(64, 8 * di.cus);
+        int waves = std::max(64, 16 * di.cus);
         while (waves > 64 && (double)waves * per_wave > 4.0 * (1 << 30)) waves /= 2;
         d->ovf_grid = waves;
         d->gstore_bytes = (size_t)waves * per_wave;
