@@ -254,17 +254,28 @@ def timeline_summary(tl, tpm, names):
 IEP_PATTERNS = {"house": "0100110110010110110010100", "c5": "0100110100010100010110010"}
 
 
+def graph_desc(name, roots=None):
+    """The workload description of make_graph's graph (and of a root shard
+    of it), as the IEP line and its profiles name it."""
+    if name.startswith("rmat"):
+        desc = "R-MAT scale %s, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 20" % name[4:]
+    elif name.startswith("cl"):
+        desc = "Chung-Lu, %s vertices, mean degree 24, exponent 2.1, degree cap 50,000, seed 21" % name[2:]
+    else:
+        raise SystemExit("unknown graph %r (rmatS or clN)" % name)
+    if roots:
+        desc += "; roots [%d, %d) of the degree-ordered graph (raw sums)" % tuple(roots)
+    return desc
+
+
 def make_graph(gat, name):
     """rmatS: R-MAT scale S, edge factor 16, (.57,.19,.19,.05), seed 20;
     clN: Chung-Lu, N vertices, mean degree 24, exponent 2.1, degree cap
     50,000, seed 21."""
+    desc = graph_desc(name)
     if name.startswith("rmat"):
-        return gat.Graph.rmat(int(name[4:]), 16, 0.57, 0.19, 0.19, 20), \
-            "R-MAT scale %s, edge factor 16, (a,b,c,d)=(.57,.19,.19,.05), seed 20" % name[4:]
-    if name.startswith("cl"):
-        return gat.Graph.chung_lu(int(name[2:]), 24.0, 2.1, 50000, 21), \
-            "Chung-Lu, %s vertices, mean degree 24, exponent 2.1, degree cap 50,000, seed 21" % name[2:]
-    raise SystemExit("unknown graph %r (rmatS or clN)" % name)
+        return gat.Graph.rmat(int(name[4:]), 16, 0.57, 0.19, 0.19, 20), desc
+    return gat.Graph.chung_lu(int(name[2:]), 24.0, 2.1, 50000, 21), desc
 
 
 class _Shard:
@@ -324,7 +335,7 @@ def iep_line(gat, args):
         plan = _Shard(plan, roots)
         off = g.csr()[0]
         E = int(off[roots[1]] - off[roots[0]])  # directed edges leaving the shard's roots
-        desc += "; roots [%d, %d) of the degree-ordered graph (raw sums)" % roots
+        desc = graph_desc(args.graph, roots)
     for _ in range(args.warmup):
         plan.count(cfg)
     _, acc = plan.count(cfg, account=True)
