@@ -11,6 +11,7 @@ of the ranges, is one GPU's share when a count is split across GPUs.
 
     python -m graphautotune_amd.sharded --graph cl4194304 --pattern house
     python -m graphautotune_amd.sharded --graph rmat22 --batched-set --estimate
+    python -m graphautotune_amd.sharded --combine part_a.json part_b.json
 """
 import argparse
 import json
@@ -222,6 +223,33 @@ def stratified_estimate(plan, config=None, per_stratum=8, min_block_s=1.0, seed=
             "seconds_se": float(np.sqrt(sec_var)), "unit": unit, "strata": strata}
 
 
+def combine(parts):
+    """Counts from the --roots outputs of several calls (or GPUs) over one
+    graph and pattern list: the root ranges must partition [0, vertices)
+    (in any order), then the raw sums are added, divided by each pattern's
+    divisor and multiplied by its multiplier, as gat_count reports them.
+    Returns {"counts", "raw", "device_seconds", "roots"}."""
+    if not parts:
+        raise ValueError("no parts to combine")
+    first = parts[0]
+    for q in parts[1:]:
+        for k in ("graph", "vertices", "patterns", "mode"):
+            if q.get(k) != first.get(k):
+                raise ValueError("parts differ in %s: %r vs %r" % (k, q.get(k), first.get(k)))
+    ranges = sorted(tuple(q["roots"]) for q in parts)
+    if ranges[0][0] != 0 or ranges[-1][1] != first["vertices"] or \
+            any(a[1] != b[0] for a, b in zip(ranges, ranges[1:])):
+        raise ValueError("root ranges %s do not partition [0, %d)" % (ranges, first["vertices"]))
+    raw = [sum(int(q["raw"][i]) for q in parts) for i in range(len(first["raw"]))]
+    counts = []
+    for r, info in zip(raw, first["count_info"]):
+        if r % info["divisor"]:
+            raise ValueError("raw sum %d is not a multiple of the divisor %d" % (r, info["divisor"]))
+        counts.append(r // info["divisor"] * info["multiplier"])
+    return {"counts": counts, "raw": raw, "device_seconds": sum(q["device_seconds"] for q in parts),
+            "roots": [list(r) for r in ranges]}
+
+
 def make_graph(spec):
     """A workload graph by name (workloads.graph: erN, rmatS with seed S,
     clN) or a path to a .bin graph."""
@@ -232,7 +260,19 @@ def make_graph(spec):
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    if argv is None:
+        argv = sys.argv[1:]
+    if argv and argv[0] == "--combine":
+        # python -m graphautotune_amd.sharded --combine a.json b.json ...
+        parts = []
+        for path in argv[1:]:
+            with open(path) as f:
+                parts.append(json.loads(f.read().strip().splitlines()[-1]))
+        print(json.dumps(combine(parts)), flush=True)
+        return
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0],
+                                 epilog="--combine A.json B.json ...: counts from the --roots outputs of a "
+                                        "partition of the roots (see combine)")
     ap.add_argument("--graph", required=True, help="rmatS, clN or a .bin file")
     ap.add_argument("--pattern", action="append", default=[],
                     help="adjacency string or one of %s (repeatable)" % ", ".join(sorted(PATTERNS)))

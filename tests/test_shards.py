@@ -231,3 +231,37 @@ def test_head_ladder_and_unit_range_estimate():
     assert est["strata"][0]["units"][0] == lo and est["strata"][-1]["units"][1] == int(plan.off[-1])
     sampled = [c for c in plan.edge_calls[len(probes) + 1:] if c[1] > c[0]]  # not the empty edge_shards checks
     assert sampled and min(c[0] for c in sampled) >= lo
+
+
+def test_combine_partition_of_root_ranges(tmp_path):
+    """--combine adds the raw sums of --roots outputs that partition the
+    roots (in any order), divides once and multiplies; gaps, overlaps,
+    mismatched workloads and raw sums off the divisor are refused."""
+    import json
+
+    from graphautotune_amd.sharded import combine, main
+    info = [{"divisor": 2, "multiplier": 3, "relabeled": True, "restrictions": []},
+            {"divisor": 1, "multiplier": 1, "relabeled": True, "restrictions": []}]
+    base = {"graph": "g", "vertices": 10, "patterns": ["p", "q"], "mode": "unique", "count_info": info}
+    a = dict(base, roots=[4, 10], raw=[6, 5], device_seconds=1.5)
+    b = dict(base, roots=[0, 4], raw=[4, 7], device_seconds=0.5)
+    got = combine([a, b])
+    assert got["counts"] == [15, 12] and got["raw"] == [10, 12] and got["device_seconds"] == 2.0
+    assert got["roots"] == [[0, 4], [4, 10]]
+    with pytest.raises(ValueError):
+        combine([a])  # [4, 10) alone leaves a gap
+    with pytest.raises(ValueError):
+        combine([a, dict(b, roots=[0, 5])])  # overlap
+    with pytest.raises(ValueError):
+        combine([a, dict(b, patterns=["p"])])
+    with pytest.raises(ValueError):
+        combine([a, dict(b, raw=[5, 7])])  # 11 is odd
+    pa, pb = tmp_path / "a.json", tmp_path / "b.json"
+    pa.write_text("log line\n" + json.dumps(a) + "\n")
+    pb.write_text(json.dumps(b))
+    import contextlib
+    import io
+    out = io.StringIO()
+    with contextlib.redirect_stdout(out):
+        main(["--combine", str(pa), str(pb)])
+    assert json.loads(out.getvalue())["counts"] == [15, 12]
