@@ -147,3 +147,49 @@ def test_config_study_helpers():
     names = {p.name: p for p in space.params}
     assert 64 not in names["search_linear_threshold"].values and "vertex_store" not in names
     assert all(c.values["vertex_store"] == 1 for c in space.random_configs(20))
+
+
+def test_study_shard_picking_and_probe(monkeypatch):
+    """pick_shard grows a root shard from one root until the default config
+    takes the target, moves to lighter roots when one root is too heavy and
+    to heavier ones when all roots from lo on are too light; the Driver's
+    probe scores a config whose first eighth already takes its limit by
+    that time scaled to the shard, without the full run."""
+    import numpy as np
+
+    from graphautotune_amd.tuning import config_study as cs
+    from graphautotune_amd.tuning import driver as drv_mod
+
+    n = 1 << 16
+    # root r costs cost[r] ms: heavy below n/16 and at n/16 itself, light far out
+    cost = np.where(np.arange(n) < n // 8, 200.0, 0.05)
+    calls = []
+
+    class StubPlan:
+        def __init__(self, graph, patterns, mode="unique", **kw):
+            self.scale = kw.get("scale", 1.0)
+
+        def count_shard(self, lo, hi, cfg=None):
+            calls.append((lo, hi, dict(cfg or {})))
+            slow = 50.0 if (cfg or {}).get("queue_chunk") == 64 else 1.0
+            return [hi - lo], {"seconds": slow * float(cost[lo:hi].sum()) / 1e3}
+
+    graph = type("G", (), {"v_cnt": lambda self: n})()
+    monkeypatch.setattr(cs.core, "Plan", StubPlan)
+    lo, hi, t = cs.pick_shard(graph, "p", target_ms=40.0)
+    assert lo == n // 4 and t * 1e3 >= 40.0 and t * 1e3 <= 80.0 + 1e-9  # one root at n/16 took 200 ms
+    assert (hi - lo) * 0.05 == pytest.approx(t * 1e3)
+    cost[:] = 0.0001  # every root light: all roots from n/16 on stay under target/4 ...
+    lo2, hi2, t2 = cs.pick_shard(graph, "p", target_ms=40.0)
+    assert lo2 < n // 16 and hi2 == n  # ... so the shard moved to heavier (lower) roots
+
+    monkeypatch.setattr(drv_mod.core, "Plan", StubPlan)
+    cost[:] = 0.1
+    d = drv_mod.Driver(graph, ["p"], reps=1, warmup=0, roots=(0, 800))
+    td = d.run({})
+    d.probe = (100, 2.5 * td)
+    calls.clear()
+    t_slow = d.run({"queue_chunk": 64})  # 50x slower: the first 100 roots take 0.5 s > 2.5 x 80 ms
+    assert calls == [(0, 100, {"queue_chunk": 64})] and t_slow == pytest.approx(50 * 0.1 * 800 / 1e3)
+    calls.clear()
+    assert d.run({"queue_chunk": 4}) == pytest.approx(td) and len(calls) == 2  # probe, then the full shard
